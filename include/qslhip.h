@@ -1,0 +1,174 @@
+/*
+ * qslhip.h -- C ABI of the MI355X (gfx950) state-vector backend for qsl.
+ *
+ * Every entry point returns an int status (QSLHIP_OK == 0) and never throws.
+ * A handle owns one state vector in device memory and one HIP stream; calls
+ * on one handle are stream-ordered, and every call that returns host data
+ * synchronises that stream first.  Handles are not thread-safe: one caller
+ * thread per handle, as with qsl::Qubits (the reference is not thread-safe).
+ *
+ * Amplitudes are interleaved {re, im} of the handle's precision (prec 8 =
+ * double, 4 = float): the layout of qsl::complex<Fp>
+ * (include/qsl/utils/complex.hpp:40-49), so get/set are plain copies.
+ *
+ * Each declaration names the reference interface it replaces.  The C++20
+ * class templates qsl::Qubits<qsl::Type::Hip, Fp> and
+ * qsl::Qubits<qsl::Type::HipNP, Fp> (include/qsl/qubits/hip.hpp,
+ * include/qsl/qubits/hip_np.hpp) are the reference-shaped API over it.
+ */
+#ifndef QSLHIP_H
+#define QSLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+    QSLHIP_OK = 0,
+    QSLHIP_EINVAL = 1,  /* bad argument (qubit index, kind, pointer, size)   */
+    QSLHIP_ESIZE = 2,   /* state size mismatch: the reference's logic_error  */
+    QSLHIP_ENOTNP = 3,  /* state not number preserving / bad nones           */
+    QSLHIP_ENOMEM = 4,  /* device allocation failed                          */
+    QSLHIP_ENODEV = 5,  /* no usable HIP device                              */
+    QSLHIP_EDEVICE = 6, /* any other HIP runtime error                       */
+    QSLHIP_EKIND = 7    /* gate not available on this handle type            */
+};
+
+/* ---- gate kinds ------------------------------------------------------------
+ * One-qubit: include/qsl/qubits/default.hpp:208-333.
+ * Two-qubit: include/qsl/qubits/default.hpp:354-608; for controlled gates q0
+ * is the control and q1 the target; for swap/fswap/np* q0, q1 are q1, q2. */
+enum {
+    QSLHIP_PAULI_X = 0, QSLHIP_PAULI_Y = 1, QSLHIP_PAULI_Z = 2,
+    QSLHIP_HADAMARD = 3, QSLHIP_PHASE = 4, QSLHIP_ROT_X = 5,
+    QSLHIP_ROT_Y = 6, QSLHIP_ROT_Z = 7,
+    QSLHIP_CNOT = 16, QSLHIP_CY = 17, QSLHIP_CZ = 18, QSLHIP_CROT_X = 19,
+    QSLHIP_CROT_Y = 20, QSLHIP_CROT_Z = 21, QSLHIP_CPHASE = 22,
+    QSLHIP_CHADAMARD = 23, QSLHIP_SWAP = 24, QSLHIP_FSWAP = 25,
+    QSLHIP_NP_ROT_X = 26, QSLHIP_NP_ROT_Y = 27, QSLHIP_NP_HADAMARD = 28
+};
+
+/* One gate of a gate list (24 bytes). */
+typedef struct {
+    int32_t kind;
+    uint32_t q0;
+    uint32_t q1;   /* unused by one-qubit gates */
+    uint32_t pad;  /* must be 0 */
+    double angle;  /* unused by fixed gates; rounded to the handle's Fp */
+} qslhip_op;
+
+/* Execution modes of qslhip_apply_ops. */
+enum {
+    QSLHIP_MODE_PER_GATE = 0, /* one kernel per gate (the reference's loop)  */
+    QSLHIP_MODE_FUSED = 1     /* gates packed into LDS-tile passes; same
+                                 per-amplitude arithmetic, bit-identical     */
+};
+
+typedef struct qslhip_state *qslhip_t;
+
+/* ---- library --------------------------------------------------------------- */
+const char *qslhip_status_string(int status);
+/* Detail of the most recent failure on the calling thread ("" if none). */
+const char *qslhip_last_error(void);
+/* Number of visible HIP devices; QSLHIP_ENODEV when the runtime has none. */
+int qslhip_device_count(int *count);
+
+/* ---- lifecycle ---------------------------------------------------------------
+ * qslhip_create    <- Qubits<Default,Fp>(unsigned nqubits)  default.hpp:114,
+ *                     default.cpp:39-46 (allocates 2^n, resets to |0..0>).
+ * qslhip_create_np <- Qubits<NP,Fp>(unsigned nqubits, unsigned nones)
+ *                     np.hpp:107, np.cpp:42-55 (throws unless 1<=nones<n);
+ *                     the device keeps only the C(n,nones) amplitudes of
+ *                     weight nones, in colex-rank order.
+ * qslhip_clone     <- copy constructor default.hpp:137 (device-to-device).
+ * qslhip_assign    <- operator= default.cpp:94-105 (QSLHIP_ESIZE when the
+ *                     qubit counts differ).
+ * prec is 8 (double) or 4 (float). */
+int qslhip_create(unsigned nqubits, int prec, qslhip_t *out);
+int qslhip_create_np(unsigned nqubits, unsigned nones, int prec, qslhip_t *out);
+int qslhip_destroy(qslhip_t h);
+int qslhip_clone(qslhip_t src, qslhip_t *out);
+int qslhip_assign(qslhip_t dst, qslhip_t src);
+/* nones is 0 for a full-vector handle; namps is the device amplitude count. */
+int qslhip_info(qslhip_t h, unsigned *nqubits, unsigned *nones, int *prec,
+                uint64_t *namps);
+/* Use `stream` (a hipStream_t, or NULL for the handle's own) for all later
+ * work of this handle.  The caller keeps the stream alive. */
+int qslhip_set_stream(qslhip_t h, void *stream);
+int qslhip_sync(qslhip_t h);
+
+/* ---- state -------------------------------------------------------------------
+ * reset      <- reset()  default.cpp:57-65 / np.cpp:68-79 (index 2^nones-1)
+ * set_state  <- setState() default.cpp:67-78; count must be 2^n
+ *               (QSLHIP_ESIZE otherwise).  NP: the full 2^n vector; its
+ *               weight is re-derived as checkStateNP does (np.cpp:81-98).
+ * get_state  <- getState() default.cpp:108-112; NP returns the full 2^n
+ *               vector (np.cpp:135-139).
+ * set_basis  <- setBasisState() default.cpp:80-90; NP recomputes nones by
+ *               popcount (np.cpp:100-116).
+ * set_num_ones <- setNumOnes() np.cpp:155-165 (resets the state).
+ * get_amps / set_amps: raw device-layout access ([first, first+count) of the
+ *               namps amplitudes; compact colex order for NP handles). */
+int qslhip_reset(qslhip_t h);
+int qslhip_set_state(qslhip_t h, const void *host, uint64_t count);
+int qslhip_get_state(qslhip_t h, void *host, uint64_t count);
+int qslhip_set_basis(qslhip_t h, uint64_t index);
+int qslhip_set_num_ones(qslhip_t h, unsigned nones);
+int qslhip_get_amps(qslhip_t h, void *host, uint64_t first, uint64_t count);
+int qslhip_set_amps(qslhip_t h, const void *host, uint64_t first,
+                    uint64_t count);
+
+/* ---- gates -------------------------------------------------------------------
+ * qslhip_gate1 <- the one-qubit members default.hpp:208-333 (rotateX/Y/Z,
+ *                 pauliX/Y/Z, hadamard, phase) and np.hpp:158-168.
+ * qslhip_gate2 <- the two-qubit members default.hpp:354-608 and
+ *                 np.hpp:173-208.
+ * The angle's cos/sin are taken on the host in the handle's Fp exactly as
+ * the reference does (gates.cpp:124-125), so both sides multiply by the
+ * same coefficients. */
+int qslhip_gate1(qslhip_t h, int kind, unsigned targ, double angle);
+int qslhip_gate2(qslhip_t h, int kind, unsigned q0, unsigned q1, double angle);
+/* A gate list applied in order (no reference counterpart: the reference
+ * calls one member per gate).  mode: QSLHIP_MODE_*. */
+int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops,
+                     int mode);
+
+/* ---- measurement ---------------------------------------------------------------
+ * Uniforms are injected (the reference draws them from an unseedable
+ * mt19937, random.cpp:32-45); a handle of precision 4 compares them as
+ * float, as the reference's Fp uniform does.
+ * prob       <- prob(targ, outcome) measure.cpp:126-147 (FP64 accumulation)
+ * collapse   <- collapse() measure.cpp:31-57 (private in the reference)
+ * measure    <- measure(targ) measure.cpp:81-107
+ * postselect <- postselect() measure.cpp:149-161
+ * measure_all<- measureAll() measure.cpp:110-123
+ * sample_all <- sampleAll(N) measure.cpp:220-234: one outcome per uniform
+ *               (the C++ layer folds them into the std::map)
+ * sample     <- sample(targ, N) measure.cpp:191-216: counts[0], counts[1]
+ * norm       <- norm(getState()) quantum.cpp:53-64 (FP64) */
+int qslhip_prob(qslhip_t h, unsigned targ, unsigned outcome, double *p);
+int qslhip_collapse(qslhip_t h, unsigned targ, unsigned outcome,
+                    double factor);
+int qslhip_measure(qslhip_t h, unsigned targ, double u, int *outcome);
+int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p);
+int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome);
+int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots,
+                      uint64_t *out);
+int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
+                  uint64_t *counts);
+int qslhip_norm(qslhip_t h, double *norm);
+
+/* ---- synthetic workloads and seeded uniforms (no reference counterpart;
+ * the reference's generators are unseedable, src/utils/random.cpp:32-45) --
+ * qslhip_uniforms: mt19937_64 seeded with splitmix64(seed), u =
+ * (x >> 11) * 2^-53 in [0, 1). */
+int qslhip_uniforms(uint64_t seed, uint64_t count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,129 @@
+"""Loader for libqslhip.so (the HIP backend's C ABI, include/qslhip.h).
+
+The library is built in-tree by __graft_entry__.build() (make -C
+qsl_amd/csrc).  There is no fallback: if the library or a HIP device is
+missing, calls fail loudly with QslHipError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqslhip.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "qslhip.h")
+
+# status codes
+OK, EINVAL, ESIZE, ENOTNP, ENOMEM, ENODEV, EDEVICE, EKIND = range(8)
+
+# gate kinds
+PAULI_X, PAULI_Y, PAULI_Z, HADAMARD, PHASE, ROT_X, ROT_Y, ROT_Z = range(8)
+CNOT, CY, CZ, CROT_X, CROT_Y, CROT_Z, CPHASE, CHADAMARD = range(16, 24)
+SWAP, FSWAP, NP_ROT_X, NP_ROT_Y, NP_HADAMARD = range(24, 29)
+
+MODE_PER_GATE, MODE_FUSED = 0, 1
+
+OP_DTYPE = np.dtype([("kind", "<i4"), ("q0", "<u4"), ("q1", "<u4"),
+                     ("pad", "<u4"), ("angle", "<f8")])
+
+
+class QslHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class SizeError(QslHipError, ValueError):
+    """The reference's std::logic_error for size mismatches."""
+
+
+class NotNPError(QslHipError, ValueError):
+    """The reference's std::logic_error for nones / non-NP states."""
+
+
+_lib = None
+
+_vp = C.c_void_p
+_u64 = C.c_uint64
+_U = C.c_uint
+_I = C.c_int
+_D = C.c_double
+
+SIGNATURES = {
+    "qslhip_status_string": (C.c_char_p, [_I]),
+    "qslhip_last_error": (C.c_char_p, []),
+    "qslhip_device_count": (_I, [C.POINTER(_I)]),
+    "qslhip_create": (_I, [_U, _I, C.POINTER(_vp)]),
+    "qslhip_create_np": (_I, [_U, _U, _I, C.POINTER(_vp)]),
+    "qslhip_destroy": (_I, [_vp]),
+    "qslhip_clone": (_I, [_vp, C.POINTER(_vp)]),
+    "qslhip_assign": (_I, [_vp, _vp]),
+    "qslhip_info": (_I, [_vp, C.POINTER(_U), C.POINTER(_U), C.POINTER(_I),
+                         C.POINTER(_u64)]),
+    "qslhip_set_stream": (_I, [_vp, _vp]),
+    "qslhip_sync": (_I, [_vp]),
+    "qslhip_reset": (_I, [_vp]),
+    "qslhip_set_state": (_I, [_vp, _vp, _u64]),
+    "qslhip_get_state": (_I, [_vp, _vp, _u64]),
+    "qslhip_set_basis": (_I, [_vp, _u64]),
+    "qslhip_set_num_ones": (_I, [_vp, _U]),
+    "qslhip_get_amps": (_I, [_vp, _vp, _u64, _u64]),
+    "qslhip_set_amps": (_I, [_vp, _vp, _u64, _u64]),
+    "qslhip_gate1": (_I, [_vp, _I, _U, _D]),
+    "qslhip_gate2": (_I, [_vp, _I, _U, _U, _D]),
+    "qslhip_apply_ops": (_I, [_vp, _vp, _u64, _I]),
+    "qslhip_prob": (_I, [_vp, _U, _U, C.POINTER(_D)]),
+    "qslhip_collapse": (_I, [_vp, _U, _U, _D]),
+    "qslhip_measure": (_I, [_vp, _U, _D, C.POINTER(_I)]),
+    "qslhip_postselect": (_I, [_vp, _U, _U, C.POINTER(_D)]),
+    "qslhip_measure_all": (_I, [_vp, _D, C.POINTER(_u64)]),
+    "qslhip_sample_all": (_I, [_vp, _vp, _u64, _vp]),
+    "qslhip_sample": (_I, [_vp, _U, _vp, _u64, _vp]),
+    "qslhip_norm": (_I, [_vp, C.POINTER(_D)]),
+    "qslhip_uniforms": (_I, [_u64, _u64, _vp]),
+}
+
+
+def lib():
+    """Load libqslhip.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QslHipError(EDEVICE, f"{LIB_PATH} is not built: run "
+                              "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc == OK:
+        return
+    L = lib()
+    msg = L.qslhip_last_error().decode() or L.qslhip_status_string(rc).decode()
+    if rc == ESIZE:
+        raise SizeError(rc, msg)
+    if rc == ENOTNP:
+        raise NotNPError(rc, msg)
+    raise QslHipError(rc, f"{L.qslhip_status_string(rc).decode()}: {msg}")
+
+
+def ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+def uniforms(seed, count):
+    """mt19937_64(splitmix64(seed)) uniforms (x >> 11) * 2^-53 in [0, 1)."""
+    out = np.empty(count, np.float64)
+    check(lib().qslhip_uniforms(seed, count, ptr(out)))
+    return out
+
+
+def device_count():
+    n = _I(0)
+    rc = lib().qslhip_device_count(C.byref(n))
+    return n.value if rc == OK else 0

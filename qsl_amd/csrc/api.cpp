@@ -1,0 +1,711 @@
+// api.cpp -- the extern "C" boundary (include/qslhip.h).
+//
+// Every entry point validates its arguments on the host before any device
+// work, maps the reference's std::logic_error cases to QSLHIP_ESIZE /
+// QSLHIP_ENOTNP / QSLHIP_EINVAL, and turns HIP failures into
+// QSLHIP_ENOMEM / QSLHIP_ENODEV / QSLHIP_EDEVICE with the detail kept in a
+// thread-local string (qslhip_last_error).  No exception crosses the ABI.
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "gate_math.h"
+
+using namespace qslhip;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        g_last_error.clear();
+        f();
+        return QSLHIP_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return QSLHIP_ENOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return QSLHIP_EDEVICE;
+    } catch (...) {
+        g_last_error = "unknown failure";
+        return QSLHIP_EDEVICE;
+    }
+}
+
+void require_handle(qslhip_t h)
+{
+    QH_REQUIRE(h != nullptr && h->amps != nullptr, QSLHIP_EINVAL, "null handle");
+}
+
+int first_device()
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1)
+        throw Error(QSLHIP_ENODEV, std::string("no HIP device: ") +
+                                       (e != hipSuccess ? hipGetErrorString(e)
+                                                        : "device count is 0"));
+    int dev = 0;
+    QH_CHECK(hipGetDevice(&dev));
+    return dev;
+}
+
+uint64_t choose_u64(unsigned n, unsigned k)
+{
+    if (k > n) return 0;
+    uint64_t a = 1;
+    for (unsigned i = 1; i <= k; i++) {
+        a *= (uint64_t)(n - k + i);
+        a /= i;
+    }
+    return a;
+}
+
+qslhip_state *alloc_handle(unsigned n, int prec, bool np, unsigned k)
+{
+    int dev = first_device();
+    QH_CHECK(hipSetDevice(dev));
+    auto *h = new qslhip_state();
+    h->device = dev;
+    h->n = n;
+    h->prec = prec;
+    h->np = np;
+    h->k = k;
+    h->namps = np ? choose_u64(n, k) : (1ull << n);
+    try {
+        QH_CHECK(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+        h->stream = h->own_stream;
+        QH_CHECK(hipEventCreateWithFlags(&h->staged, hipEventDisableTiming));
+        size_t bytes = h->namps * h->amp_bytes();
+        hipError_t e = hipMalloc(&h->amps, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            h->amps = nullptr;
+            throw Error(QSLHIP_ENOMEM, "hipMalloc of " + std::to_string(bytes) +
+                                           " bytes for the state failed: " +
+                                           hipGetErrorString(e));
+        }
+        if (np) np_init(h);
+    } catch (...) {
+        qslhip_destroy(h);
+        throw;
+    }
+    return h;
+}
+
+void reset_impl(qslhip_t h)
+{
+    // Default: |0...0> (default.cpp:57-65).  NP: the lowest weight-k string
+    // 2^k - 1, whose colex rank is 0 (np.cpp:68-79).
+    launch_fill_basis(h, 0);
+}
+
+template <typename Fp>
+void check_np_host(const void *host, uint64_t count, unsigned *nones)
+{
+    // checkStateNP (src/utils/quantum.cpp:133-156): every nonzero
+    // amplitude must carry the same Hamming weight.
+    const Fp *p = static_cast<const Fp *>(host);
+    bool found = false;
+    unsigned w0 = 0;
+    for (uint64_t i = 0; i < count; i++) {
+        Fp a = p[2 * i] * p[2 * i] + p[2 * i + 1] * p[2 * i + 1];
+        if (a != 0) {
+            unsigned w = (unsigned)__builtin_popcountll(i);
+            if (!found) {
+                w0 = w;
+                found = true;
+            } else if (w != w0) {
+                throw Error(QSLHIP_ENOTNP, "Input state is not number preserving.");
+            }
+        }
+    }
+    *nones = w0;
+}
+
+}  // namespace
+
+namespace qslhip {
+
+void DevBuf::ensure(size_t b)
+{
+    if (b <= bytes) return;
+    release();
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ptr = nullptr;
+        bytes = 0;
+        throw Error(QSLHIP_ENOMEM, "hipMalloc of " + std::to_string(b) +
+                                       " scratch bytes failed");
+    }
+    bytes = b;
+}
+
+void DevBuf::release()
+{
+    if (ptr) {
+        // frees wait for every queued kernel that may still read the buffer
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr);
+    }
+    ptr = nullptr;
+    bytes = 0;
+}
+
+void PinnedBuf::ensure(size_t b)
+{
+    if (b <= bytes) return;
+    release();
+    QH_CHECK(hipHostMalloc(&ptr, b, hipHostMallocDefault));
+    bytes = b;
+}
+
+void PinnedBuf::release()
+{
+    if (ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipHostFree(ptr);
+    }
+    ptr = nullptr;
+    bytes = 0;
+}
+
+Tuning &tuning()
+{
+    static Tuning t;
+    return t;
+}
+
+}  // namespace qslhip
+
+extern "C" {
+
+const char *qslhip_status_string(int status)
+{
+    switch (status) {
+    case QSLHIP_OK: return "ok";
+    case QSLHIP_EINVAL: return "invalid argument";
+    case QSLHIP_ESIZE: return "state size mismatch";
+    case QSLHIP_ENOTNP: return "state is not number preserving";
+    case QSLHIP_ENOMEM: return "device out of memory";
+    case QSLHIP_ENODEV: return "no HIP device";
+    case QSLHIP_EDEVICE: return "HIP runtime error";
+    case QSLHIP_EKIND: return "gate not available on this simulator type";
+    default: return "unknown status";
+    }
+}
+
+const char *qslhip_last_error(void) { return g_last_error.c_str(); }
+
+int qslhip_device_count(int *count)
+{
+    return guarded([&] {
+        QH_REQUIRE(count, QSLHIP_EINVAL, "null count");
+        *count = 0;
+        hipError_t e = hipGetDeviceCount(count);
+        if (e != hipSuccess) {
+            *count = 0;
+            throw Error(QSLHIP_ENODEV, hipGetErrorString(e));
+        }
+        if (*count < 1) throw Error(QSLHIP_ENODEV, "device count is 0");
+    });
+}
+
+int qslhip_create(unsigned nqubits, int prec, qslhip_t *out)
+{
+    return guarded([&] {
+        QH_REQUIRE(out, QSLHIP_EINVAL, "null output handle");
+        *out = nullptr;
+        QH_REQUIRE(prec == 4 || prec == 8, QSLHIP_EINVAL, "prec must be 4 or 8");
+        QH_REQUIRE(nqubits >= 1 && nqubits <= kMaxQubits, QSLHIP_EINVAL,
+                   "nqubits must be in [1, 40]");
+        qslhip_t h = alloc_handle(nqubits, prec, false, 0);
+        reset_impl(h);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *out = h;
+    });
+}
+
+int qslhip_create_np(unsigned nqubits, unsigned nones, int prec, qslhip_t *out)
+{
+    return guarded([&] {
+        QH_REQUIRE(out, QSLHIP_EINVAL, "null output handle");
+        *out = nullptr;
+        QH_REQUIRE(prec == 4 || prec == 8, QSLHIP_EINVAL, "prec must be 4 or 8");
+        QH_REQUIRE(nqubits >= 2 && nqubits <= 62, QSLHIP_EINVAL,
+                   "nqubits must be in [2, 62]");
+        // np.cpp:48-50
+        QH_REQUIRE(nones >= 1 && nones < nqubits, QSLHIP_ENOTNP,
+                   "The number of ones must be 1 <= nones < nqubits.");
+        QH_REQUIRE(choose_u64(nqubits, nones) <= (1ull << 36), QSLHIP_EINVAL,
+                   "C(n, nones) too large for one device");
+        qslhip_t h = alloc_handle(nqubits, prec, true, nones);
+        reset_impl(h);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *out = h;
+    });
+}
+
+int qslhip_destroy(qslhip_t h)
+{
+    return guarded([&] {
+        if (!h) return;
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        if (h->np) np_free(h);
+        h->red.release();
+        h->cdf.release();
+        h->shots.release();
+        h->optab.release();
+        h->pin.release();
+        if (h->amps) (void)hipFree(h->amps);
+        if (h->staged) (void)hipEventDestroy(h->staged);
+        if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+        delete h;
+    });
+}
+
+int qslhip_clone(qslhip_t src, qslhip_t *out)
+{
+    return guarded([&] {
+        require_handle(src);
+        QH_REQUIRE(out, QSLHIP_EINVAL, "null output handle");
+        *out = nullptr;
+        qslhip_t h = alloc_handle(src->n, src->prec, src->np, src->k);
+        QH_CHECK(hipStreamSynchronize(src->stream));
+        QH_CHECK(hipMemcpyAsync(h->amps, src->amps, src->namps * src->amp_bytes(),
+                                hipMemcpyDeviceToDevice, h->stream));
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *out = h;
+    });
+}
+
+int qslhip_assign(qslhip_t dst, qslhip_t src)
+{
+    return guarded([&] {
+        require_handle(dst);
+        require_handle(src);
+        if (dst == src) return;
+        // default.cpp:94-105 / np.cpp:119-131
+        QH_REQUIRE(dst->n == src->n, QSLHIP_ESIZE,
+                   "Cannot assign Qubits object vector from different number of qubits");
+        QH_REQUIRE(dst->np == src->np && dst->prec == src->prec, QSLHIP_EINVAL,
+                   "handles differ in type or precision");
+        if (dst->np && dst->k != src->k) {
+            // the NP reference copies the full vector, whatever its weight
+            np_free(dst);
+            (void)hipFree(dst->amps);
+            dst->amps = nullptr;
+            dst->k = src->k;
+            dst->namps = src->namps;
+            QH_CHECK(hipMalloc(&dst->amps, dst->namps * dst->amp_bytes()));
+            np_init(dst);
+        }
+        QH_CHECK(hipStreamSynchronize(src->stream));
+        QH_CHECK(hipMemcpyAsync(dst->amps, src->amps, src->namps * src->amp_bytes(),
+                                hipMemcpyDeviceToDevice, dst->stream));
+        QH_CHECK(hipStreamSynchronize(dst->stream));
+    });
+}
+
+int qslhip_info(qslhip_t h, unsigned *nqubits, unsigned *nones, int *prec,
+                uint64_t *namps)
+{
+    return guarded([&] {
+        require_handle(h);
+        if (nqubits) *nqubits = h->n;
+        if (nones) *nones = h->np ? h->k : 0;
+        if (prec) *prec = h->prec;
+        if (namps) *namps = h->namps;
+    });
+}
+
+int qslhip_set_stream(qslhip_t h, void *stream)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        h->stream = stream ? static_cast<hipStream_t>(stream) : h->own_stream;
+    });
+}
+
+int qslhip_sync(qslhip_t h)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int qslhip_reset(qslhip_t h)
+{
+    return guarded([&] {
+        require_handle(h);
+        reset_impl(h);
+    });
+}
+
+int qslhip_set_state(qslhip_t h, const void *host, uint64_t count)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(host, QSLHIP_EINVAL, "null host pointer");
+        // default.cpp:70-74
+        QH_REQUIRE(count == (1ull << h->n), QSLHIP_ESIZE,
+                   "Cannot assign state vector from different number of qubits");
+        if (!h->np) {
+            QH_CHECK(hipMemcpyAsync(h->amps, host, count * h->amp_bytes(),
+                                    hipMemcpyHostToDevice, h->stream));
+            QH_CHECK(hipStreamSynchronize(h->stream));
+            return;
+        }
+        // NP: re-derive nones (np.cpp:81-98), then gather the weight-k
+        // amplitudes into the compact layout on the device.
+        unsigned k = 0;
+        if (h->prec == 8) check_np_host<double>(host, count, &k);
+        else check_np_host<float>(host, count, &k);
+        QH_REQUIRE(k >= 1 && k < h->n, QSLHIP_ENOTNP,
+                   "The number of ones must be 1 <= nones < nqubits.");
+        if (k != h->k) {
+            np_free(h);
+            (void)hipFree(h->amps);
+            h->amps = nullptr;
+            h->k = k;
+            h->namps = choose_u64(h->n, k);
+            QH_CHECK(hipMalloc(&h->amps, h->namps * h->amp_bytes()));
+            np_init(h);
+        }
+        DevBuf full;
+        full.ensure(count * h->amp_bytes());
+        QH_CHECK(hipMemcpyAsync(full.ptr, host, count * h->amp_bytes(),
+                                hipMemcpyHostToDevice, h->stream));
+        np_gather_full(h, full.ptr);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        full.release();
+    });
+}
+
+int qslhip_get_state(qslhip_t h, void *host, uint64_t count)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(host, QSLHIP_EINVAL, "null host pointer");
+        QH_REQUIRE(count == (1ull << h->n), QSLHIP_ESIZE, "count must be 2^nqubits");
+        if (!h->np) {
+            QH_CHECK(hipMemcpyAsync(host, h->amps, count * h->amp_bytes(),
+                                    hipMemcpyDeviceToHost, h->stream));
+            QH_CHECK(hipStreamSynchronize(h->stream));
+            return;
+        }
+        DevBuf full;
+        full.ensure(count * h->amp_bytes());
+        np_scatter_full(h, full.ptr);
+        QH_CHECK(hipMemcpyAsync(host, full.ptr, count * h->amp_bytes(),
+                                hipMemcpyDeviceToHost, h->stream));
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        full.release();
+    });
+}
+
+int qslhip_get_amps(qslhip_t h, void *host, uint64_t first, uint64_t count)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(host || count == 0, QSLHIP_EINVAL, "null host pointer");
+        QH_REQUIRE(first <= h->namps && count <= h->namps - first, QSLHIP_EINVAL,
+                   "amplitude range out of bounds");
+        QH_CHECK(hipMemcpyAsync(host,
+                                static_cast<char *>(h->amps) + first * h->amp_bytes(),
+                                count * h->amp_bytes(), hipMemcpyDeviceToHost,
+                                h->stream));
+        QH_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int qslhip_set_amps(qslhip_t h, const void *host, uint64_t first, uint64_t count)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(host || count == 0, QSLHIP_EINVAL, "null host pointer");
+        QH_REQUIRE(first <= h->namps && count <= h->namps - first, QSLHIP_EINVAL,
+                   "amplitude range out of bounds");
+        QH_CHECK(hipMemcpyAsync(static_cast<char *>(h->amps) + first * h->amp_bytes(),
+                                host, count * h->amp_bytes(), hipMemcpyHostToDevice,
+                                h->stream));
+        QH_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int qslhip_set_basis(qslhip_t h, uint64_t index)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(h->n >= 64 || index < (1ull << h->n), QSLHIP_EINVAL,
+                   "basis index out of range");
+        if (!h->np) {
+            launch_fill_basis(h, index);
+            return;
+        }
+        // np.cpp:100-116: nones follows the index's popcount
+        unsigned k = (unsigned)__builtin_popcountll(index);
+        QH_REQUIRE(k >= 1 && k < h->n, QSLHIP_ENOTNP,
+                   "The number of ones must be 1 <= nones < nqubits.");
+        if (k != h->k) {
+            QH_CHECK(hipStreamSynchronize(h->stream));
+            np_free(h);
+            (void)hipFree(h->amps);
+            h->amps = nullptr;
+            h->k = k;
+            h->namps = choose_u64(h->n, k);
+            QH_CHECK(hipMalloc(&h->amps, h->namps * h->amp_bytes()));
+            np_init(h);
+        }
+        launch_fill_basis(h, np_rank_host(index));
+    });
+}
+
+int qslhip_set_num_ones(qslhip_t h, unsigned nones)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(h->np, QSLHIP_EKIND, "setNumOnes needs a number-preserving handle");
+        QH_REQUIRE(nones >= 1 && nones < h->n, QSLHIP_ENOTNP,
+                   "The number of ones must be 1 <= nones < nqubits.");
+        QH_REQUIRE(choose_u64(h->n, nones) <= (1ull << 36), QSLHIP_EINVAL,
+                   "C(n, nones) too large for one device");
+        if (nones != h->k) {
+            QH_CHECK(hipStreamSynchronize(h->stream));
+            np_free(h);
+            (void)hipFree(h->amps);
+            h->amps = nullptr;
+            h->k = nones;
+            h->namps = choose_u64(h->n, nones);
+            QH_CHECK(hipMalloc(&h->amps, h->namps * h->amp_bytes()));
+            np_init(h);
+        }
+        reset_impl(h);
+    });
+}
+
+static void validate_gate(qslhip_t h, int kind, unsigned q0, unsigned q1)
+{
+    if (is_gate1(kind)) {
+        QH_REQUIRE(q0 < h->n, QSLHIP_EINVAL, "qubit index out of range");
+    } else if (is_gate2(kind)) {
+        QH_REQUIRE(q0 < h->n && q1 < h->n, QSLHIP_EINVAL, "qubit index out of range");
+        QH_REQUIRE(q0 != q1, QSLHIP_EINVAL, "two-qubit gate on one qubit");
+    } else {
+        throw Error(QSLHIP_EINVAL, "unknown gate kind " + std::to_string(kind));
+    }
+    if (h->np)
+        QH_REQUIRE(is_np_kind(kind), QSLHIP_EKIND,
+                   "gate does not preserve the number of ones");
+}
+
+int qslhip_gate1(qslhip_t h, int kind, unsigned targ, double angle)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(is_gate1(kind), QSLHIP_EINVAL, "not a one-qubit gate kind");
+        validate_gate(h, kind, targ, 0);
+        GateDesc g = make_desc(h->prec, kind, targ, 0, angle);
+        if (h->np) np_launch_gate(h, g);
+        else launch_gate(h, g);
+    });
+}
+
+int qslhip_gate2(qslhip_t h, int kind, unsigned q0, unsigned q1, double angle)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(is_gate2(kind), QSLHIP_EINVAL, "not a two-qubit gate kind");
+        validate_gate(h, kind, q0, q1);
+        GateDesc g = make_desc(h->prec, kind, q0, q1, angle);
+        if (h->np) np_launch_gate(h, g);
+        else launch_gate(h, g);
+    });
+}
+
+int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
+        QH_REQUIRE(mode == QSLHIP_MODE_PER_GATE || mode == QSLHIP_MODE_FUSED,
+                   QSLHIP_EINVAL, "unknown mode");
+        std::vector<GateDesc> gates;
+        gates.reserve(nops);
+        for (uint64_t i = 0; i < nops; i++) {
+            const qslhip_op &o = ops[i];
+            validate_gate(h, o.kind, o.q0, o.q1);
+            gates.push_back(make_desc(h->prec, o.kind, o.q0,
+                                      is_gate2(o.kind) ? o.q1 : 0, o.angle));
+        }
+        if (mode == QSLHIP_MODE_PER_GATE || h->np) {
+            for (const GateDesc &g : gates) {
+                if (h->np) np_launch_gate(h, g);
+                else launch_gate(h, g);
+            }
+            return;
+        }
+        const Tuning &tu = tuning();
+        int logt = h->prec == 8 ? tu.logt_f64 : tu.logt_f32;
+        int low = h->prec == 8 ? tu.low_f64 : tu.low_f32;
+        std::vector<TilePass> passes;
+        std::vector<TileOp> tops;
+        plan_tile_passes(h->n, h->prec, gates, logt, low, passes, tops);
+        launch_tile_passes(h, passes, tops);
+    });
+}
+
+int qslhip_prob(qslhip_t h, unsigned targ, unsigned outcome, double *p)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(p, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
+        *p = h->np ? np_prob(h, targ, outcome) : device_prob(h, targ, outcome);
+    });
+}
+
+int qslhip_collapse(qslhip_t h, unsigned targ, unsigned outcome, double factor)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
+        if (h->np) np_collapse(h, targ, outcome, factor);
+        else launch_collapse(h, targ, outcome, factor);
+    });
+}
+
+// u is compared in the handle's Fp, as the reference's `Fp rand`.
+static double as_fp(qslhip_t h, double u)
+{
+    return h->prec == 8 ? u : (double)(float)u;
+}
+
+int qslhip_measure(qslhip_t h, unsigned targ, double u, int *outcome)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(outcome, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(targ < h->n, QSLHIP_EINVAL, "qubit index out of range");
+        // measure.cpp:81-107, with prob0 and the factor in FP64
+        double prob0 = h->np ? np_prob(h, targ, 0) : device_prob(h, targ, 0);
+        double uu = as_fp(h, u);
+        int o;
+        double factor;
+        if (uu < prob0) {
+            o = 0;
+            factor = 1.0 / std::sqrt(prob0);
+        } else {
+            o = 1;
+            factor = 1.0 / std::sqrt(1.0 - prob0);
+        }
+        if (h->np) np_collapse(h, targ, (unsigned)o, factor);
+        else launch_collapse(h, targ, (unsigned)o, factor);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *outcome = o;
+    });
+}
+
+int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(p, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
+        // measure.cpp:149-161
+        double pr = h->np ? np_prob(h, targ, outcome) : device_prob(h, targ, outcome);
+        double factor = 1.0 / std::sqrt(pr);
+        if (h->np) np_collapse(h, targ, outcome, factor);
+        else launch_collapse(h, targ, outcome, factor);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *p = pr;
+    });
+}
+
+int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots, uint64_t *out)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE((u && out) || nshots == 0, QSLHIP_EINVAL, "null array");
+        std::vector<double> uf;
+        const double *src = u;
+        if (h->prec == 4 && nshots) {
+            uf.assign(u, u + nshots);
+            for (double &x : uf) x = (double)(float)x;
+            src = uf.data();
+        }
+        if (h->np) np_sample(h, src, nshots, out);
+        else device_sample(h, src, nshots, out);
+    });
+}
+
+int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(outcome, QSLHIP_EINVAL, "null output");
+        double uu = as_fp(h, u);
+        uint64_t o = 0;
+        if (h->np) np_sample(h, &uu, 1, &o);
+        else device_sample(h, &uu, 1, &o);
+        // measure.cpp:110-123: collapse onto the outcome
+        if (h->np) launch_fill_basis(h, np_rank_host(o));
+        else launch_fill_basis(h, o);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *outcome = o;
+    });
+}
+
+int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
+                  uint64_t *counts)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(counts && (u || nshots == 0), QSLHIP_EINVAL, "null array");
+        QH_REQUIRE(targ < h->n, QSLHIP_EINVAL, "qubit index out of range");
+        // measure.cpp:191-216: one prob, then u < prob0 counts as 0
+        double prob0 = h->np ? np_prob(h, targ, 0) : device_prob(h, targ, 0);
+        uint64_t c0 = 0;
+        for (uint64_t i = 0; i < nshots; i++)
+            if (as_fp(h, u[i]) < prob0) c0++;
+        counts[0] = c0;
+        counts[1] = nshots - c0;
+    });
+}
+
+int qslhip_norm(qslhip_t h, double *norm)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(norm, QSLHIP_EINVAL, "null output");
+        double s = h->np ? np_prob(h, 0, 0) + np_prob(h, 0, 1) : device_norm2(h);
+        *norm = std::sqrt(s);
+    });
+}
+
+int qslhip_uniforms(uint64_t seed, uint64_t count, double *out)
+{
+    return guarded([&] {
+        QH_REQUIRE(out || count == 0, QSLHIP_EINVAL, "null output");
+        // splitmix64 of the seed (Steele, Lea & Flood 2014)
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        std::mt19937_64 gen(z);
+        for (uint64_t i = 0; i < count; i++)
+            out[i] = (double)(gen() >> 11) * 0x1.0p-53;
+    });
+}
+
+}  // extern "C"

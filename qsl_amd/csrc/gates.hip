@@ -1,0 +1,237 @@
+// gates.hip -- one kernel per gate: the streaming path.
+//
+// A gate on qubit t pairs amplitude i0 = ins0(f, t) with i0 | 2^t for the
+// flat pair id f in [0, 2^(n-1)) -- the mask form of the reference's s/r
+// loop nest (src/qubits/default/gates.cpp:35-41).  Consecutive lanes take
+// consecutive f, so for t >= 2 (FP64) both streams are contiguous runs of
+// whole cache lines; targets below the LDS threshold go through a one-op
+// LDS tile pass instead (fused.hip), which loads the tile with fully
+// contiguous 16-byte lanes.  Two-qubit gates insert two zero bits
+// (f in [0, 2^(n-2))).  Each lane handles kItems pairs, issuing all loads
+// before any arithmetic so that every wave keeps 2*kItems dwordx4 (FP64)
+// loads in flight.
+#include "gate_math.h"
+
+namespace qslhip {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 4;
+
+// One-qubit gate acting on both amplitudes of each pair.
+template <typename T, int BODY>
+__global__ __launch_bounds__(kBlock) void k_pair1(cplx<T> *__restrict__ st,
+                                                  uint64_t npairs, unsigned t,
+                                                  T c, T s)
+{
+    const uint64_t tb = 1ull << t;
+    uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
+    cplx<T> a[kItems], b[kItems];
+    uint64_t i0[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        uint64_t f = f0 + (uint64_t)j * kBlock;
+        i0[j] = ins0(f, t);
+        if (f < npairs) {
+            a[j] = st[i0[j]];
+            b[j] = st[i0[j] | tb];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; j++) pair_update<T>(BODY, a[j], b[j], c, s);
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        uint64_t f = f0 + (uint64_t)j * kBlock;
+        if (f < npairs) {
+            st[i0[j]] = a[j];
+            st[i0[j] | tb] = b[j];
+        }
+    }
+}
+
+// One-qubit gate touching only the |1> half (pauliZ, phase).
+template <typename T, int BODY>
+__global__ __launch_bounds__(kBlock) void k_one1(cplx<T> *__restrict__ st,
+                                                 uint64_t npairs, unsigned t,
+                                                 T c, T s)
+{
+    const uint64_t tb = 1ull << t;
+    uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
+    cplx<T> b[kItems];
+    uint64_t i1[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        uint64_t f = f0 + (uint64_t)j * kBlock;
+        i1[j] = ins0(f, t) | tb;
+        if (f < npairs) b[j] = st[i1[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; j++) one_update<T>(BODY, b[j], c, s);
+#pragma unroll
+    for (int j = 0; j < kItems; j++) {
+        uint64_t f = f0 + (uint64_t)j * kBlock;
+        if (f < npairs) st[i1[j]] = b[j];
+    }
+}
+
+// Two-qubit gates: f in [0, 2^(n-2)) -> base with zeros at lo < hi.
+//   CLASS PC_CTRL:     pair (base|abit, base|abit|bbit)     a = ctrl
+//   CLASS PC_SWAPLIKE: pair (base|abit, base|bbit)
+//   CLASS PC_FSWAP:    swap (base|abit, base|bbit), negate base|abit|bbit
+//   CLASS PC_DIAG2:    one (base|abit|bbit)
+template <typename T, int CLASS, int BODY>
+__global__ __launch_bounds__(kBlock) void k_gate2(cplx<T> *__restrict__ st,
+                                                  uint64_t nquads, unsigned lo,
+                                                  unsigned hi, uint64_t abit,
+                                                  uint64_t bbit, T c, T s)
+{
+    uint64_t f0 = (uint64_t)blockIdx.x * (kBlock * kItems) + threadIdx.x;
+    uint64_t base[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; j++)
+        base[j] = ins0(ins0(f0 + (uint64_t)j * kBlock, lo), hi);
+    if constexpr (CLASS == PC_DIAG2) {
+        cplx<T> x[kItems];
+#pragma unroll
+        for (int j = 0; j < kItems; j++)
+            if (f0 + (uint64_t)j * kBlock < nquads) x[j] = st[base[j] | abit | bbit];
+#pragma unroll
+        for (int j = 0; j < kItems; j++) one_update<T>(BODY, x[j], c, s);
+#pragma unroll
+        for (int j = 0; j < kItems; j++)
+            if (f0 + (uint64_t)j * kBlock < nquads) st[base[j] | abit | bbit] = x[j];
+    } else {
+        const uint64_t off_a = abit;
+        const uint64_t off_b = (CLASS == PC_CTRL) ? (abit | bbit) : bbit;
+        cplx<T> a[kItems], b[kItems], z[kItems];
+#pragma unroll
+        for (int j = 0; j < kItems; j++) {
+            if (f0 + (uint64_t)j * kBlock < nquads) {
+                a[j] = st[base[j] | off_a];
+                b[j] = st[base[j] | off_b];
+                if constexpr (CLASS == PC_FSWAP) z[j] = st[base[j] | abit | bbit];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; j++) {
+            pair_update<T>(BODY, a[j], b[j], c, s);
+            if constexpr (CLASS == PC_FSWAP)
+                one_update<T>(QSLHIP_PAULI_Z, z[j], c, s);
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; j++) {
+            if (f0 + (uint64_t)j * kBlock < nquads) {
+                st[base[j] | off_a] = a[j];
+                st[base[j] | off_b] = b[j];
+                if constexpr (CLASS == PC_FSWAP) st[base[j] | abit | bbit] = z[j];
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_basis(cplx<T> *__restrict__ st,
+                                                       uint64_t namps,
+                                                       uint64_t index)
+{
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; i < namps; i += (uint64_t)gridDim.x * kBlock) {
+        cplx<T> v;
+        v.re = (i == index) ? (T)1 : (T)0;
+        v.im = (T)0;
+        st[i] = v;
+    }
+}
+
+template <typename T>
+void dispatch(qslhip_state *h, const GateDesc &g)
+{
+    cplx<T> *st = static_cast<cplx<T> *>(h->amps);
+    const T c = (T)g.c, s = (T)g.s;
+    const unsigned n = h->n;
+    const unsigned per = kBlock * kItems;
+    if (is_gate1(g.kind)) {
+        uint64_t np = 1ull << (n - 1);
+        dim3 grid(grid_for(np, per)), blk(kBlock);
+        switch (g.kind) {
+#define P1(K)                                                                  \
+    case K:                                                                    \
+        hipLaunchKernelGGL((k_pair1<T, body_of(K)>), grid, blk, 0, h->stream,  \
+                           st, np, g.q0, c, s);                                \
+        break;
+            P1(QSLHIP_PAULI_X) P1(QSLHIP_PAULI_Y) P1(QSLHIP_HADAMARD)
+            P1(QSLHIP_ROT_X) P1(QSLHIP_ROT_Y) P1(QSLHIP_ROT_Z)
+#undef P1
+        case QSLHIP_PAULI_Z:
+            hipLaunchKernelGGL((k_one1<T, QSLHIP_PAULI_Z>), grid, blk, 0,
+                               h->stream, st, np, g.q0, c, s);
+            break;
+        case QSLHIP_PHASE:
+            hipLaunchKernelGGL((k_one1<T, QSLHIP_PHASE>), grid, blk, 0,
+                               h->stream, st, np, g.q0, c, s);
+            break;
+        default:
+            throw Error(QSLHIP_EINVAL, "unknown one-qubit gate kind");
+        }
+    } else {
+        uint64_t nq = 1ull << (n - 2);
+        unsigned lo = g.q0 < g.q1 ? g.q0 : g.q1, hi = g.q0 < g.q1 ? g.q1 : g.q0;
+        uint64_t abit = 1ull << g.q0, bbit = 1ull << g.q1;
+        dim3 grid(grid_for(nq, per)), blk(kBlock);
+        switch (g.kind) {
+#define P2(K)                                                                  \
+    case K:                                                                    \
+        hipLaunchKernelGGL((k_gate2<T, pair_class(K), body_of(K)>), grid, blk, \
+                           0, h->stream, st, nq, lo, hi, abit, bbit, c, s);    \
+        break;
+            P2(QSLHIP_CNOT) P2(QSLHIP_CY) P2(QSLHIP_CZ) P2(QSLHIP_CROT_X)
+            P2(QSLHIP_CROT_Y) P2(QSLHIP_CROT_Z) P2(QSLHIP_CPHASE)
+            P2(QSLHIP_CHADAMARD) P2(QSLHIP_SWAP) P2(QSLHIP_FSWAP)
+            P2(QSLHIP_NP_ROT_X) P2(QSLHIP_NP_ROT_Y) P2(QSLHIP_NP_HADAMARD)
+#undef P2
+        default:
+            throw Error(QSLHIP_EINVAL, "unknown two-qubit gate kind");
+        }
+    }
+    QH_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+// Low targets go through a one-op LDS tile; the rest stream.
+void launch_gate(qslhip_state *h, const GateDesc &g)
+{
+    const Tuning &tu = tuning();
+    const int lim = h->prec == 8 ? tu.lds_max_target_f64 : tu.lds_max_target_f32;
+    const int logt = h->prec == 8 ? tu.logt_f64 : tu.logt_f32;
+    unsigned top = g.q0;
+    if (is_gate2(g.kind)) top = g.q0 > g.q1 ? g.q0 : g.q1;
+    if ((int)top <= lim && (int)h->n >= logt) {
+        std::vector<GateDesc> one{g};
+        std::vector<TilePass> passes;
+        std::vector<TileOp> ops;
+        plan_tile_passes(h->n, h->prec, one, logt, logt, passes, ops);
+        launch_tile_passes(h, passes, ops);
+        return;
+    }
+    if (h->n < 2 && is_gate2(g.kind)) throw Error(QSLHIP_EINVAL, "need 2 qubits");
+    if (h->prec == 8) dispatch<double>(h, g);
+    else dispatch<float>(h, g);
+}
+
+void launch_fill_basis(qslhip_state *h, uint64_t index)
+{
+    unsigned grid = grid_for(h->namps, kBlock);
+    if (grid > 65536u * 4) grid = 65536u * 4;
+    if (h->prec == 8)
+        hipLaunchKernelGGL(k_fill_basis<double>, dim3(grid), dim3(kBlock), 0,
+                           h->stream, static_cast<cplx<double> *>(h->amps),
+                           h->namps, index);
+    else
+        hipLaunchKernelGGL(k_fill_basis<float>, dim3(grid), dim3(kBlock), 0,
+                           h->stream, static_cast<cplx<float> *>(h->amps),
+                           h->namps, index);
+    QH_CHECK(hipGetLastError());
+}
+
+}  // namespace qslhip

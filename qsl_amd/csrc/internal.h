@@ -1,0 +1,164 @@
+// internal.h -- shared declarations of the qslhip library (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qslhip.h"
+
+namespace qslhip {
+
+// Thrown inside the library, turned into a status code at the C boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define QH_CHECK(expr)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess)                                                 \
+            throw ::qslhip::Error(e_ == hipErrorOutOfMemory ? QSLHIP_ENOMEM   \
+                                                            : QSLHIP_EDEVICE, \
+                                  std::string(#expr) + ": " +                 \
+                                      hipGetErrorString(e_));                 \
+    } while (0)
+
+#define QH_REQUIRE(cond, code, msg)                                           \
+    do {                                                                      \
+        if (!(cond)) throw ::qslhip::Error((code), (msg));                    \
+    } while (0)
+
+// Largest qubit count a handle accepts: 2^36 FP32 amplitudes = 512 GiB is
+// beyond one MI355X (288 GB), so allocation fails first; the bound keeps
+// every shift in range.
+constexpr unsigned kMaxQubits = 40;
+
+// Device scratch that grows on demand and is released with the handle.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b);
+    void release();
+};
+
+// Host pinned scratch (results of reductions, staging of gate tables).
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t b);
+    void release();
+};
+
+// Complex amplitude of precision T: {re, im}, same bytes as qsl::complex<T>.
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+    T re, im;
+};
+
+// A gate prepared for the device: kind, qubits, coefficients in the
+// handle's precision (already rounded from the host Fp computation).
+struct GateDesc {
+    int kind;
+    unsigned q0, q1;
+    double c, s;
+};
+
+// One op inside an LDS tile pass (see fused.hip).  pa/pb are tile bit
+// positions of q0/q1, or -1 when the qubit is outside the tile (then its
+// value is fixed per group and read from the group's base index).
+struct TileOp {
+    int32_t kind;
+    int32_t pa, pb;
+    uint32_t q0, q1;
+    uint32_t pad;
+    double c, s;
+};
+
+// A planned tile pass: which qubits are in the tile, and a slice of the
+// circuit's TileOp table.
+struct TilePass {
+    int logt;              // tile has 2^logt amplitudes
+    uint64_t tile_mask;    // qubits in the tile
+    uint32_t op_first, op_count;
+};
+
+struct NPTables;  // np_compact.hip
+
+}  // namespace qslhip
+
+struct qslhip_state {
+    int device = 0;
+    unsigned n = 0;        // qubits
+    unsigned k = 0;        // number of ones (NP handles)
+    bool np = false;       // compact number-preserving layout
+    int prec = 8;          // 8 double, 4 float
+    uint64_t namps = 0;    // device amplitudes
+    void *amps = nullptr;  // device state
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    qslhip::DevBuf red;      // reduction partials + result slots
+    qslhip::DevBuf cdf;      // sampling prefix sums
+    qslhip::DevBuf shots;    // per-shot uniforms / outcomes
+    qslhip::DevBuf optab;    // tile-pass op tables
+    qslhip::PinnedBuf pin;   // pinned host staging
+    hipEvent_t staged = nullptr;  // last copy out of `pin` has completed
+    qslhip::NPTables *npt = nullptr;
+    size_t amp_bytes() const { return prec == 8 ? 16 : 8; }
+};
+
+namespace qslhip {
+
+// ---- launchers (defined in the .hip files) -----------------------------------
+void launch_gate(qslhip_state *h, const GateDesc &g);
+void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
+                        const std::vector<TileOp> &ops);
+void launch_fill_basis(qslhip_state *h, uint64_t index);
+double device_prob(qslhip_state *h, unsigned t, unsigned outcome);
+double device_norm2(qslhip_state *h);
+void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome,
+                     double factor);
+void device_sample(qslhip_state *h, const double *u, uint64_t nshots,
+                   uint64_t *out);
+
+// NP compact backend (np_compact.hip)
+void np_init(qslhip_state *h);
+void np_free(qslhip_state *h);
+void np_launch_gate(qslhip_state *h, const GateDesc &g);
+double np_prob(qslhip_state *h, unsigned t, unsigned outcome);
+void np_collapse(qslhip_state *h, unsigned t, unsigned outcome, double factor);
+void np_sample(qslhip_state *h, const double *u, uint64_t nshots,
+               uint64_t *out);
+void np_scatter_full(qslhip_state *h, void *dev_full);
+void np_gather_full(qslhip_state *h, const void *dev_full);
+uint64_t np_rank_host(uint64_t x);
+
+// planner (plan.cpp)
+void plan_tile_passes(unsigned n, int prec, const std::vector<GateDesc> &gates,
+                      int logt, int low_bits, std::vector<TilePass> &passes,
+                      std::vector<TileOp> &ops);
+GateDesc make_desc(int prec, int kind, unsigned q0, unsigned q1, double angle);
+bool is_gate1(int kind);
+bool is_gate2(int kind);
+bool is_np_kind(int kind);
+
+// Tunables picked by the autotuner (autotune.cpp), with safe defaults.
+struct Tuning {
+    int logt_f64 = 10, low_f64 = 4;
+    int logt_f32 = 11, low_f32 = 5;
+    int lds_max_target_f64 = 3;  // per-gate path: targets below use a tile
+    int lds_max_target_f32 = 4;
+};
+Tuning &tuning();
+
+inline unsigned grid_for(uint64_t items, unsigned per_block)
+{
+    uint64_t g = (items + per_block - 1) / per_block;
+    return (unsigned)(g ? g : 1);
+}
+
+}  // namespace qslhip

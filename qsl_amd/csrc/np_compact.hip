@@ -1,0 +1,566 @@
+// np_compact.hip -- number-preserving backend on the compact colex layout.
+//
+// The reference's NP backend keeps a full 2^n vector and walks fixed-weight
+// lookup lists (src/qubits/np/np.cpp:168-181).  Here the device holds only
+// the C(n,k) amplitudes of weight k, at index rank(x) where
+//     rank(x) = sum over set bits p_1 < ... < p_k of C(p_i, i),
+// the position of x in the reference's own enumeration order (Gosper's next
+// visits weight-k strings in increasing numeric order, which is colex
+// order; src/utils/misc.cpp:44-90).
+//
+// Two-level rank: x = (hi << L) | lo with L = min(n, max(16, n - 16)) low
+// bits.  Strings sharing `hi` are contiguous ("segment hi"), ordered by lo
+// within it, so
+//     rank(x) = seg_off[hi] + rank_lo[lo],   unrank = table lookups,
+// with rank_lo / lo_list (the L-bit strings of each weight, in increasing
+// order) and seg_off (prefix sums of C(L, k - popcount(hi))) built exactly
+// on the host in u64.  A workgroup handles up to kChunk consecutive ranks of
+// one segment (the block map), so every element's string costs one
+// coalesced table load and the amplitude loads are contiguous.  Gates whose
+// qubits are all in `hi` are constant per segment: whole workgroups skip,
+// and a swap-like pair in `hi` pairs element i of segment hi with element i
+// of segment hi', two contiguous streams.
+#include <cstring>
+
+#include "gate_math.h"
+
+namespace qslhip {
+
+struct NPBlk {
+    uint32_t hi;  // segment (high bits of the strings)
+    uint32_t i0;  // first index inside the segment
+};
+
+struct NPTables {
+    unsigned L = 0, hib = 0;
+    uint64_t *seg_off = nullptr;      // [2^hib + 1] rank of each segment's start
+    uint32_t *lo_list = nullptr;      // [2^L] strings grouped by weight, ascending
+    uint32_t *rank_lo = nullptr;      // [2^L] index of a string inside its weight
+    uint32_t *class_start = nullptr;  // [L + 2] offset of each weight in lo_list
+    NPBlk *blk = nullptr;             // block map
+    uint32_t nblk = 0;
+    std::vector<uint64_t> h_seg_off;  // host copies
+    std::vector<uint32_t> h_lo_list, h_rank_lo, h_class_start;
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 4;
+constexpr int kChunk = kBlock * kItems;
+
+uint64_t choose_host(unsigned n, unsigned k)
+{
+    if (k > n) return 0;
+    uint64_t a = 1;
+    for (unsigned i = 1; i <= k; i++) {
+        a *= (uint64_t)(n - k + i);
+        a /= i;
+    }
+    return a;
+}
+
+// Device view passed by value to every kernel.
+struct NPDev {
+    const uint64_t *seg_off;
+    const uint32_t *lo_list;
+    const uint32_t *rank_lo;
+    const uint32_t *class_start;
+    const NPBlk *blk;
+    unsigned L, k;
+    uint64_t lomask;
+};
+
+NPDev dev_view(const qslhip_state *h)
+{
+    const NPTables *t = h->npt;
+    NPDev d;
+    d.seg_off = t->seg_off;
+    d.lo_list = t->lo_list;
+    d.rank_lo = t->rank_lo;
+    d.class_start = t->class_start;
+    d.blk = t->blk;
+    d.L = t->L;
+    d.k = h->k;
+    d.lomask = (1ull << t->L) - 1;
+    return d;
+}
+
+__device__ __forceinline__ uint64_t np_rank(const NPDev &d, uint64_t x)
+{
+    return d.seg_off[x >> d.L] + d.rank_lo[x & d.lomask];
+}
+
+// Per-block segment context.
+struct Seg {
+    uint32_t hi, i0, size, cstart;
+    uint64_t off;
+};
+
+__device__ __forceinline__ Seg seg_of(const NPDev &d)
+{
+    Seg s;
+    NPBlk b = d.blk[blockIdx.x];
+    s.hi = b.hi;
+    s.i0 = b.i0;
+    unsigned w = d.k - __popc(b.hi);
+    s.cstart = d.class_start[w];
+    s.size = d.class_start[w + 1] - s.cstart;
+    s.off = d.seg_off[b.hi];
+    return s;
+}
+
+// Element-wise gate classes on the compact vector (the same per-amplitude
+// arithmetic as the reference NP gates, src/qubits/np/gates.cpp):
+//   PC_ONE   phase / pauliZ on t       (np/gates.cpp:33-81)
+//   PC_PAIR1 rotateZ on t, per-element halves (np/gates.cpp:83-113)
+//   PC_DIAG2 controlZ / controlPhase   (np/gates.cpp:118-150, 195-233)
+//   PC_CTRL  controlRotateZ            (np/gates.cpp:153-192)
+//   PC_SWAPLIKE / PC_FSWAP  owner x (bit q0 = 1, bit q1 = 0) updates itself
+//            and its partner x ^ bit q0 ^ bit q1 (np/gates.cpp:236-415);
+//            fswap also negates the |11> strings (swap then controlZ).
+template <typename T, int CLASS>
+__global__ __launch_bounds__(kBlock) void k_np_gate(cplx<T> *__restrict__ amp, NPDev d,
+                                                    int body, unsigned q0, unsigned q1,
+                                                    T c, T s)
+{
+    const Seg g = seg_of(d);
+    const uint64_t b0 = 1ull << q0, b1 = 1ull << q1;
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+    // Whole-segment decisions when the qubits are in the high part.
+    if (CLASS == PC_ONE && q0 >= d.L && !(hix & b0)) return;
+    if (CLASS == PC_DIAG2 && ((q0 >= d.L && !(hix & b0)) || (q1 >= d.L && !(hix & b1))))
+        return;
+    if (CLASS == PC_CTRL && q0 >= d.L && !(hix & b0)) return;
+    if ((CLASS == PC_SWAPLIKE) && q0 >= d.L && q1 >= d.L && !((hix & b0) && !(hix & b1)))
+        return;
+#pragma unroll
+    for (int e = 0; e < kItems; e++) {
+        uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        if (i >= g.size || i >= g.i0 + kChunk) continue;
+        uint64_t x = hix | d.lo_list[g.cstart + i];
+        uint64_t r = g.off + i;
+        if constexpr (CLASS == PC_ONE) {
+            if (x & b0) {
+                cplx<T> a = amp[r];
+                one_update<T>(body, a, c, s);
+                amp[r] = a;
+            }
+        } else if constexpr (CLASS == PC_PAIR1) {
+            cplx<T> a = amp[r];
+            rz_half<T>((x & b0) != 0, a, c, s);
+            amp[r] = a;
+        } else if constexpr (CLASS == PC_DIAG2) {
+            if ((x & b0) && (x & b1)) {
+                cplx<T> a = amp[r];
+                one_update<T>(body, a, c, s);
+                amp[r] = a;
+            }
+        } else if constexpr (CLASS == PC_CTRL) {
+            if (x & b0) {
+                cplx<T> a = amp[r];
+                rz_half<T>((x & b1) != 0, a, c, s);
+                amp[r] = a;
+            }
+        } else {  // PC_SWAPLIKE, PC_FSWAP
+            if ((x & b0) && !(x & b1)) {
+                uint64_t rp = np_rank(d, x ^ b0 ^ b1);
+                cplx<T> a = amp[r], b = amp[rp];
+                pair_update<T>(body, a, b, c, s);
+                amp[r] = a;
+                amp[rp] = b;
+            } else if (CLASS == PC_FSWAP && (x & b0) && (x & b1)) {
+                cplx<T> a = amp[r];
+                one_update<T>(QSLHIP_PAULI_Z, a, c, s);
+                amp[r] = a;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Per-block partial of sum |a|^2 over strings with bit t == outcome.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_prob(const cplx<T> *__restrict__ amp,
+                                                    NPDev d, unsigned t,
+                                                    unsigned outcome,
+                                                    double *__restrict__ part)
+{
+    __shared__ double sh[kBlock / 64];
+    const Seg g = seg_of(d);
+    const uint64_t tb = 1ull << t;
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+    double acc = 0;
+    bool seg_skip = (t >= d.L) && (((hix & tb) != 0) != (outcome != 0));
+    if (!seg_skip) {
+        for (int e = 0; e < kItems; e++) {
+            uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+            if (i >= g.size || i >= g.i0 + kChunk) continue;
+            uint64_t x = hix | d.lo_list[g.cstart + i];
+            if (((x & tb) != 0) == (outcome != 0)) {
+                cplx<T> a = amp[g.off + i];
+                T p = a.re * a.re + a.im * a.im;
+                acc += (double)p;
+            }
+        }
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int w = 0; w < kBlock / 64; w++) r += sh[w];
+        part[blockIdx.x] = r;
+    }
+}
+
+// Deterministic sum of n doubles into *out (one workgroup, fixed order).
+__global__ __launch_bounds__(1024) void k_sum_all(const double *__restrict__ v, uint64_t n,
+                                                  double *__restrict__ out)
+{
+    __shared__ double sh[1024];
+    double acc = 0;
+    for (uint64_t i = threadIdx.x; i < n; i += 1024) acc += v[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 512; off >= 1; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_collapse(cplx<T> *__restrict__ amp, NPDev d,
+                                                        unsigned t, unsigned outcome,
+                                                        double factor)
+{
+    const Seg g = seg_of(d);
+    const uint64_t tb = 1ull << t;
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+    for (int e = 0; e < kItems; e++) {
+        uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        if (i >= g.size || i >= g.i0 + kChunk) continue;
+        uint64_t x = hix | d.lo_list[g.cstart + i];
+        cplx<T> a = amp[g.off + i];
+        if (((x & tb) != 0) == (outcome != 0)) {
+            a.re = (T)((double)a.re * factor);
+            a.im = (T)((double)a.im * factor);
+        } else {
+            a.re = 0;
+            a.im = 0;
+        }
+        amp[g.off + i] = a;
+    }
+}
+
+// full[x] <-> amp[rank(x)]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_scatter(const cplx<T> *__restrict__ amp,
+                                                       NPDev d, cplx<T> *__restrict__ full,
+                                                       int to_full)
+{
+    const Seg g = seg_of(d);
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+    for (int e = 0; e < kItems; e++) {
+        uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        if (i >= g.size || i >= g.i0 + kChunk) continue;
+        uint64_t x = hix | d.lo_list[g.cstart + i];
+        if (to_full) full[x] = amp[g.off + i];
+        else const_cast<cplx<T> *>(amp)[g.off + i] = full[x];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_zero(cplx<T> *__restrict__ v, uint64_t n)
+{
+    uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        cplx<T> z;
+        z.re = 0;
+        z.im = 0;
+        v[i] = z;
+    }
+}
+
+// CDF search over ranks, then unrank to the basis index.
+__global__ __launch_bounds__(kBlock) void k_np_search(const double *__restrict__ cdf,
+                                                      uint64_t namps, NPDev d,
+                                                      uint64_t nseg,
+                                                      const double *__restrict__ u,
+                                                      uint64_t nshots,
+                                                      uint64_t *__restrict__ out)
+{
+    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nshots) return;
+    double uu = u[i];
+    bool zero = !(uu > 0.0);
+    uint64_t lo = 0, hi = namps;
+    while (lo < hi) {
+        uint64_t mid = lo + ((hi - lo) >> 1);
+        double v = cdf[mid];
+        bool ge = zero ? (v > 0.0) : (v >= uu);
+        if (ge) hi = mid;
+        else lo = mid + 1;
+    }
+    if (lo >= namps) {
+        out[i] = 0;
+        return;
+    }
+    // last segment whose start is <= lo
+    uint64_t a = 0, b = nseg;  // seg_off has nseg + 1 entries
+    while (b - a > 1) {
+        uint64_t m = (a + b) >> 1;
+        if (d.seg_off[m] <= lo) a = m;
+        else b = m;
+    }
+    uint64_t idx = lo - d.seg_off[a];
+    unsigned w = d.k - __popcll(a);
+    out[i] = (a << d.L) | d.lo_list[d.class_start[w] + idx];
+}
+
+template <typename T>
+void gate_impl(qslhip_state *h, const GateDesc &g)
+{
+    NPDev d = dev_view(h);
+    cplx<T> *amp = static_cast<cplx<T> *>(h->amps);
+    dim3 grid(h->npt->nblk), blk(kBlock);
+    const T c = (T)g.c, s = (T)g.s;
+    const int body = body_of(g.kind);
+    switch (g.kind) {
+    case QSLHIP_PAULI_Z: case QSLHIP_PHASE:
+        hipLaunchKernelGGL((k_np_gate<T, PC_ONE>), grid, blk, 0, h->stream, amp, d, body,
+                           g.q0, 0u, c, s);
+        break;
+    case QSLHIP_ROT_Z:
+        hipLaunchKernelGGL((k_np_gate<T, PC_PAIR1>), grid, blk, 0, h->stream, amp, d, body,
+                           g.q0, 0u, c, s);
+        break;
+    case QSLHIP_CZ: case QSLHIP_CPHASE:
+        hipLaunchKernelGGL((k_np_gate<T, PC_DIAG2>), grid, blk, 0, h->stream, amp, d, body,
+                           g.q0, g.q1, c, s);
+        break;
+    case QSLHIP_CROT_Z:
+        hipLaunchKernelGGL((k_np_gate<T, PC_CTRL>), grid, blk, 0, h->stream, amp, d, body,
+                           g.q0, g.q1, c, s);
+        break;
+    case QSLHIP_SWAP: case QSLHIP_NP_ROT_X: case QSLHIP_NP_ROT_Y:
+    case QSLHIP_NP_HADAMARD:
+        hipLaunchKernelGGL((k_np_gate<T, PC_SWAPLIKE>), grid, blk, 0, h->stream, amp, d,
+                           body, g.q0, g.q1, c, s);
+        break;
+    case QSLHIP_FSWAP:
+        hipLaunchKernelGGL((k_np_gate<T, PC_FSWAP>), grid, blk, 0, h->stream, amp, d, body,
+                           g.q0, g.q1, c, s);
+        break;
+    default:
+        throw Error(QSLHIP_EKIND, "gate does not preserve the number of ones");
+    }
+    QH_CHECK(hipGetLastError());
+}
+
+template <typename T>
+double prob_impl(qslhip_state *h, unsigned t, unsigned outcome)
+{
+    NPDev d = dev_view(h);
+    uint32_t nb = h->npt->nblk;
+    h->red.ensure(((size_t)nb + 8) * sizeof(double));
+    double *part = static_cast<double *>(h->red.ptr);
+    double *res = part + nb;
+    hipLaunchKernelGGL(k_np_prob<T>, dim3(nb), dim3(kBlock), 0, h->stream,
+                       static_cast<const cplx<T> *>(h->amps), d, t, outcome, part);
+    hipLaunchKernelGGL(k_sum_all, dim3(1), dim3(1024), 0, h->stream, part, (uint64_t)nb,
+                       res);
+    QH_CHECK(hipGetLastError());
+    h->pin.ensure(64);
+    if (h->staged) QH_CHECK(hipEventSynchronize(h->staged));
+    QH_CHECK(hipMemcpyAsync(h->pin.ptr, res, sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+    QH_CHECK(hipStreamSynchronize(h->stream));
+    double v;
+    std::memcpy(&v, h->pin.ptr, sizeof(double));
+    return v;
+}
+
+}  // namespace
+
+uint64_t np_rank_host(uint64_t x)
+{
+    uint64_t r = 0;
+    unsigned i = 0;
+    while (x) {
+        unsigned p = (unsigned)__builtin_ctzll(x);
+        i++;
+        r += choose_host(p, i);
+        x &= x - 1;
+    }
+    return r;
+}
+
+void np_init(qslhip_state *h)
+{
+    auto *t = new NPTables();
+    h->npt = t;
+    const unsigned n = h->n, k = h->k;
+    unsigned L = n <= 16 ? n : (n - 16 > 16 ? n - 16 : 16);
+    if (L > n) L = n;
+    const unsigned hib = n - L;
+    t->L = L;
+    t->hib = hib;
+    QH_REQUIRE(L <= 24 && hib <= 24, QSLHIP_EINVAL, "NP handle too large");
+    // Low-part tables.
+    const uint64_t nlo = 1ull << L;
+    t->h_class_start.assign(L + 2, 0);
+    for (uint64_t p = 0; p < nlo; p++) t->h_class_start[__builtin_popcountll(p) + 1]++;
+    for (unsigned w = 1; w < L + 2; w++) t->h_class_start[w] += t->h_class_start[w - 1];
+    t->h_lo_list.assign(nlo, 0);
+    t->h_rank_lo.assign(nlo, 0);
+    std::vector<uint32_t> fill(t->h_class_start.begin(), t->h_class_start.end() - 1);
+    for (uint64_t p = 0; p < nlo; p++) {  // increasing p: colex order per weight
+        unsigned w = (unsigned)__builtin_popcountll(p);
+        uint32_t pos = fill[w]++;
+        t->h_lo_list[pos] = (uint32_t)p;
+        t->h_rank_lo[p] = pos - t->h_class_start[w];
+    }
+    // Segment offsets: segment hi holds C(L, k - popcount(hi)) strings.
+    const uint64_t nseg = 1ull << hib;
+    t->h_seg_off.assign(nseg + 1, 0);
+    std::vector<NPBlk> blks;
+    for (uint64_t s = 0; s < nseg; s++) {
+        unsigned wh = (unsigned)__builtin_popcountll(s);
+        uint64_t size = (wh <= k && k - wh <= L) ? choose_host(L, k - wh) : 0;
+        t->h_seg_off[s + 1] = t->h_seg_off[s] + size;
+        for (uint64_t i0 = 0; i0 < size; i0 += kChunk) blks.push_back({(uint32_t)s, (uint32_t)i0});
+    }
+    QH_REQUIRE(t->h_seg_off[nseg] == h->namps, QSLHIP_EDEVICE,
+               "NP segment table does not add up to C(n,k)");
+    t->nblk = (uint32_t)blks.size();
+    QH_CHECK(hipMalloc(&t->seg_off, (nseg + 1) * sizeof(uint64_t)));
+    QH_CHECK(hipMalloc(&t->lo_list, nlo * sizeof(uint32_t)));
+    QH_CHECK(hipMalloc(&t->rank_lo, nlo * sizeof(uint32_t)));
+    QH_CHECK(hipMalloc(&t->class_start, (L + 2) * sizeof(uint32_t)));
+    QH_CHECK(hipMalloc(&t->blk, blks.size() * sizeof(NPBlk)));
+    QH_CHECK(hipMemcpy(t->seg_off, t->h_seg_off.data(), (nseg + 1) * sizeof(uint64_t),
+                       hipMemcpyHostToDevice));
+    QH_CHECK(hipMemcpy(t->lo_list, t->h_lo_list.data(), nlo * sizeof(uint32_t),
+                       hipMemcpyHostToDevice));
+    QH_CHECK(hipMemcpy(t->rank_lo, t->h_rank_lo.data(), nlo * sizeof(uint32_t),
+                       hipMemcpyHostToDevice));
+    QH_CHECK(hipMemcpy(t->class_start, t->h_class_start.data(), (L + 2) * sizeof(uint32_t),
+                       hipMemcpyHostToDevice));
+    QH_CHECK(hipMemcpy(t->blk, blks.data(), blks.size() * sizeof(NPBlk),
+                       hipMemcpyHostToDevice));
+}
+
+void np_free(qslhip_state *h)
+{
+    NPTables *t = h->npt;
+    if (!t) return;
+    (void)hipDeviceSynchronize();
+    if (t->seg_off) (void)hipFree(t->seg_off);
+    if (t->lo_list) (void)hipFree(t->lo_list);
+    if (t->rank_lo) (void)hipFree(t->rank_lo);
+    if (t->class_start) (void)hipFree(t->class_start);
+    if (t->blk) (void)hipFree(t->blk);
+    delete t;
+    h->npt = nullptr;
+}
+
+void np_launch_gate(qslhip_state *h, const GateDesc &g)
+{
+    if (h->prec == 8) gate_impl<double>(h, g);
+    else gate_impl<float>(h, g);
+}
+
+double np_prob(qslhip_state *h, unsigned t, unsigned outcome)
+{
+    return h->prec == 8 ? prob_impl<double>(h, t, outcome) : prob_impl<float>(h, t, outcome);
+}
+
+void np_collapse(qslhip_state *h, unsigned t, unsigned outcome, double factor)
+{
+    NPDev d = dev_view(h);
+    if (h->prec == 8)
+        hipLaunchKernelGGL(k_np_collapse<double>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                           h->stream, static_cast<cplx<double> *>(h->amps), d, t, outcome,
+                           factor);
+    else
+        hipLaunchKernelGGL(k_np_collapse<float>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                           h->stream, static_cast<cplx<float> *>(h->amps), d, t, outcome,
+                           factor);
+    QH_CHECK(hipGetLastError());
+}
+
+void np_scatter_full(qslhip_state *h, void *dev_full)
+{
+    NPDev d = dev_view(h);
+    uint64_t dim = 1ull << h->n;
+    unsigned zg = grid_for(dim, kBlock * 4);
+    if (h->prec == 8) {
+        hipLaunchKernelGGL(k_zero<double>, dim3(zg), dim3(kBlock), 0, h->stream,
+                           static_cast<cplx<double> *>(dev_full), dim);
+        hipLaunchKernelGGL(k_np_scatter<double>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                           h->stream, static_cast<const cplx<double> *>(h->amps), d,
+                           static_cast<cplx<double> *>(dev_full), 1);
+    } else {
+        hipLaunchKernelGGL(k_zero<float>, dim3(zg), dim3(kBlock), 0, h->stream,
+                           static_cast<cplx<float> *>(dev_full), dim);
+        hipLaunchKernelGGL(k_np_scatter<float>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                           h->stream, static_cast<const cplx<float> *>(h->amps), d,
+                           static_cast<cplx<float> *>(dev_full), 1);
+    }
+    QH_CHECK(hipGetLastError());
+}
+
+void np_gather_full(qslhip_state *h, const void *dev_full)
+{
+    NPDev d = dev_view(h);
+    if (h->prec == 8)
+        hipLaunchKernelGGL(k_np_scatter<double>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                           h->stream, static_cast<const cplx<double> *>(h->amps), d,
+                           const_cast<cplx<double> *>(static_cast<const cplx<double> *>(dev_full)),
+                           0);
+    else
+        hipLaunchKernelGGL(k_np_scatter<float>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                           h->stream, static_cast<const cplx<float> *>(h->amps), d,
+                           const_cast<cplx<float> *>(static_cast<const cplx<float> *>(dev_full)),
+                           0);
+    QH_CHECK(hipGetLastError());
+}
+
+}  // namespace qslhip
+
+// CDF over the compact vector in rank order (= increasing basis index, as
+// the reference's generateDist walk over lookup(n, k), np/measure.cpp:72-92)
+// reuses the full-vector scan kernels of measure.hip.
+namespace qslhip {
+void build_cdf_any(qslhip_state *h);
+
+void np_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out)
+{
+    build_cdf_any(h);
+    if (nshots == 0) {
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        return;
+    }
+    h->shots.ensure(nshots * 16);
+    double *du = static_cast<double *>(h->shots.ptr);
+    uint64_t *dout = reinterpret_cast<uint64_t *>(du + nshots);
+    QH_CHECK(hipMemcpyAsync(du, u, nshots * sizeof(double), hipMemcpyHostToDevice,
+                            h->stream));
+    NPDev d = dev_view(h);
+    hipLaunchKernelGGL(k_np_search, dim3(grid_for(nshots, kBlock)), dim3(kBlock), 0,
+                       h->stream, static_cast<const double *>(h->cdf.ptr), h->namps, d,
+                       (uint64_t)1 << h->npt->hib, du, nshots, dout);
+    QH_CHECK(hipGetLastError());
+    QH_CHECK(hipMemcpyAsync(out, dout, nshots * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            h->stream));
+    QH_CHECK(hipStreamSynchronize(h->stream));
+}
+}  // namespace qslhip
