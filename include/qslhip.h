@@ -99,6 +99,8 @@ int qslhip_info(qslhip_t h, unsigned *nqubits, unsigned *nones, int *prec,
 /* Use `stream` (a hipStream_t, or NULL for the handle's own) for all later
  * work of this handle.  The caller keeps the stream alive. */
 int qslhip_set_stream(qslhip_t h, void *stream);
+/* The hipStream_t all of this handle's work is queued on (for event timing). */
+int qslhip_get_stream(qslhip_t h, void **stream);
 int qslhip_sync(qslhip_t h);
 
 /* ---- state -------------------------------------------------------------------
@@ -136,6 +138,14 @@ int qslhip_gate2(qslhip_t h, int kind, unsigned q0, unsigned q1, double angle);
  * calls one member per gate).  mode: QSLHIP_MODE_*. */
 int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops,
                      int mode);
+
+/* Planner introspection (host only, no device work): how
+ * qslhip_apply_ops(QSLHIP_MODE_FUSED) would split `ops` on an n-qubit state
+ * of precision prec.  Writes up to `cap` tile-qubit masks and per-pass gate
+ * counts; *npasses is the full pass count. */
+int qslhip_plan(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
+                uint64_t *npasses, uint64_t *tile_masks, uint64_t *op_counts,
+                uint64_t cap);
 
 /* ---- measurement ---------------------------------------------------------------
  * Uniforms are injected (the reference draws them from an unseedable

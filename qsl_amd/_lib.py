@@ -61,7 +61,9 @@ SIGNATURES = {
     "qslhip_info": (_I, [_vp, C.POINTER(_U), C.POINTER(_U), C.POINTER(_I),
                          C.POINTER(_u64)]),
     "qslhip_set_stream": (_I, [_vp, _vp]),
+    "qslhip_get_stream": (_I, [_vp, C.POINTER(_vp)]),
     "qslhip_sync": (_I, [_vp]),
+    "qslhip_plan": (_I, [_U, _I, _vp, _u64, C.POINTER(_u64), _vp, _vp, _u64]),
     "qslhip_reset": (_I, [_vp]),
     "qslhip_set_state": (_I, [_vp, _vp, _u64]),
     "qslhip_get_state": (_I, [_vp, _vp, _u64]),
@@ -127,3 +129,15 @@ def device_count():
     n = _I(0)
     rc = lib().qslhip_device_count(C.byref(n))
     return n.value if rc == OK else 0
+
+
+def plan(nqubits, prec, ops, cap=4096):
+    """Tile passes qslhip_apply_ops(FUSED) would run: (masks, op_counts)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    npasses = _u64(0)
+    masks = np.zeros(cap, np.uint64)
+    counts = np.zeros(cap, np.uint64)
+    check(lib().qslhip_plan(nqubits, prec, ptr(ops), len(ops), C.byref(npasses),
+                            ptr(masks), ptr(counts), cap))
+    m = min(cap, npasses.value)
+    return masks[:m].copy(), counts[:m].copy()

@@ -336,6 +336,15 @@ int qslhip_set_stream(qslhip_t h, void *stream)
     });
 }
 
+int qslhip_get_stream(qslhip_t h, void **stream)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(stream, QSLHIP_EINVAL, "null output");
+        *stream = static_cast<void *>(h->stream);
+    });
+}
+
 int qslhip_sync(qslhip_t h)
 {
     return guarded([&] {
@@ -562,6 +571,39 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
         std::vector<TileOp> tops;
         plan_tile_passes(h->n, h->prec, gates, logt, low, passes, tops);
         launch_tile_passes(h, passes, tops);
+    });
+}
+
+int qslhip_plan(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
+                uint64_t *npasses, uint64_t *tile_masks, uint64_t *op_counts,
+                uint64_t cap)
+{
+    return guarded([&] {
+        QH_REQUIRE(npasses, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
+        QH_REQUIRE(prec == 4 || prec == 8, QSLHIP_EINVAL, "prec must be 4 or 8");
+        QH_REQUIRE(nqubits >= 1 && nqubits <= kMaxQubits, QSLHIP_EINVAL,
+                   "nqubits must be in [1, 40]");
+        std::vector<GateDesc> gates;
+        for (uint64_t i = 0; i < nops; i++) {
+            const qslhip_op &o = ops[i];
+            bool ok1 = is_gate1(o.kind) && o.q0 < nqubits;
+            bool ok2 = is_gate2(o.kind) && o.q0 < nqubits && o.q1 < nqubits && o.q0 != o.q1;
+            QH_REQUIRE(ok1 || ok2, QSLHIP_EINVAL, "bad op " + std::to_string(i));
+            gates.push_back(make_desc(prec, o.kind, o.q0, is_gate2(o.kind) ? o.q1 : 0,
+                                      o.angle));
+        }
+        const Tuning &tu = tuning();
+        int logt = prec == 8 ? tu.logt_f64 : tu.logt_f32;
+        int low = prec == 8 ? tu.low_f64 : tu.low_f32;
+        std::vector<TilePass> passes;
+        std::vector<TileOp> tops;
+        plan_tile_passes(nqubits, prec, gates, logt, low, passes, tops);
+        *npasses = passes.size();
+        for (uint64_t i = 0; i < passes.size() && i < cap; i++) {
+            if (tile_masks) tile_masks[i] = passes[i].tile_mask;
+            if (op_counts) op_counts[i] = passes[i].op_count;
+        }
     });
 }
 
