@@ -168,6 +168,12 @@ int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p);
 int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome);
 int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots,
                       uint64_t *out);
+/* sampleAll2 <- sampleAll2(N) measure.cpp:237-267: counts[x] = number of
+ *               uniforms u with cdf(x-1) <= u < cdf(x) (strict merge rule);
+ *               uniforms beyond the total mass are not counted.  ncounts
+ *               must be 2^nqubits (indexed by basis state, NP included). */
+int qslhip_sample_all2(qslhip_t h, const double *u, uint64_t nshots,
+                       uint64_t *counts, uint64_t ncounts);
 int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
                   uint64_t *counts);
 int qslhip_norm(qslhip_t h, double *norm);

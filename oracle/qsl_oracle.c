@@ -96,21 +96,25 @@ uint64_t orc_np_unrank(uint64_t r, unsigned k)
 #define RCOS cos
 #define RSIN sin
 #define RSQRT sqrt
+#define RSINCOS sincos
 #include "oracle_impl.inc"
 #undef R
 #undef S
 #undef RCOS
 #undef RSIN
 #undef RSQRT
+#undef RSINCOS
 
 #define R float
 #define S f
 #define RCOS cosf
 #define RSIN sinf
 #define RSQRT sqrtf
+#define RSINCOS sincosf
 #include "oracle_impl.inc"
 #undef R
 #undef S
 #undef RCOS
 #undef RSIN
 #undef RSQRT
+#undef RSINCOS

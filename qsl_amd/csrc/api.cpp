@@ -691,6 +691,25 @@ int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots, uint64_t *ou
     });
 }
 
+int qslhip_sample_all2(qslhip_t h, const double *u, uint64_t nshots,
+                       uint64_t *counts, uint64_t ncounts)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(counts && (u || nshots == 0), QSLHIP_EINVAL, "null array");
+        QH_REQUIRE(ncounts == (1ull << h->n), QSLHIP_ESIZE, "ncounts must be 2^nqubits");
+        std::vector<double> uf(u, u + nshots);
+        if (h->prec == 4)
+            for (double &x : uf) x = (double)(float)x;
+        std::vector<uint64_t> out(nshots);
+        if (h->np) np_sample(h, uf.data(), nshots, out.data(), true);
+        else device_sample(h, uf.data(), nshots, out.data(), true);
+        std::memset(counts, 0, ncounts * sizeof(uint64_t));
+        for (uint64_t x : out)
+            if (x != ~0ull) counts[x]++;
+    });
+}
+
 int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome)
 {
     return guarded([&] {

@@ -123,7 +123,7 @@ double device_norm2(qslhip_state *h);
 void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome,
                      double factor);
 void device_sample(qslhip_state *h, const double *u, uint64_t nshots,
-                   uint64_t *out);
+                   uint64_t *out, bool strict = false);
 
 // NP compact backend (np_compact.hip)
 void np_init(qslhip_state *h);
@@ -132,7 +132,7 @@ void np_launch_gate(qslhip_state *h, const GateDesc &g);
 double np_prob(qslhip_state *h, unsigned t, unsigned outcome);
 void np_collapse(qslhip_state *h, unsigned t, unsigned outcome, double factor);
 void np_sample(qslhip_state *h, const double *u, uint64_t nshots,
-               uint64_t *out);
+               uint64_t *out, bool strict = false);
 void np_scatter_full(qslhip_state *h, void *dev_full);
 void np_gather_full(qslhip_state *h, const void *dev_full);
 uint64_t np_rank_host(uint64_t x);

@@ -197,15 +197,18 @@ __global__ __launch_bounds__(kBlock) void k_chunk_scan(const cplx<T> *__restrict
 // First index i with cdf[i] >= u (u > 0), or with cdf[i] > 0 (u <= 0: the
 // reference would index out of bounds; this picks the first outcome of
 // nonzero probability); namps when u exceeds the total -> outcome 0.
+// With `strict`, the first index with cdf[i] > u instead: the merge rule of
+// sampleAll2 (measure.cpp:237-267, `rand < sum`).
 __device__ __forceinline__ uint64_t cdf_search(const double *__restrict__ cdf,
-                                               uint64_t namps, double u)
+                                               uint64_t namps, double u, bool strict)
 {
     uint64_t lo = 0, hi = namps;
-    const bool zero = !(u > 0.0);
+    const bool zero = strict || !(u > 0.0);
+    const double key = strict ? u : 0.0;
     while (lo < hi) {
         uint64_t mid = lo + ((hi - lo) >> 1);
         double v = cdf[mid];
-        bool ge = zero ? (v > 0.0) : (v >= u);
+        bool ge = zero ? (v > key) : (v >= u);
         if (ge) hi = mid;
         else lo = mid + 1;
     }
@@ -215,13 +218,14 @@ __device__ __forceinline__ uint64_t cdf_search(const double *__restrict__ cdf,
 __global__ __launch_bounds__(kBlock) void k_search(const double *__restrict__ cdf,
                                                    uint64_t namps,
                                                    const double *__restrict__ u,
-                                                   uint64_t nshots,
+                                                   uint64_t nshots, int strict,
                                                    uint64_t *__restrict__ out)
 {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nshots) return;
-    uint64_t r = cdf_search(cdf, namps, u[i]);
-    out[i] = (r >= namps) ? 0 : r;
+    uint64_t r = cdf_search(cdf, namps, u[i], strict != 0);
+    // past the total mass: drawSample returns 0; sampleAll2 counts nothing
+    out[i] = (r >= namps) ? (strict ? ~0ull : 0ull) : r;
 }
 
 template <typename T>
@@ -294,7 +298,8 @@ void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome, double facto
 }
 
 // out[i] = basis index sampled by uniform u[i] (host arrays).
-void device_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out)
+void device_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out,
+                   bool strict)
 {
     if (h->prec == 8) build_cdf<double>(h);
     else build_cdf<float>(h);
@@ -309,7 +314,7 @@ void device_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *
                             h->stream));
     hipLaunchKernelGGL(k_search, dim3(grid_for(nshots, kBlock)), dim3(kBlock), 0,
                        h->stream, static_cast<const double *>(h->cdf.ptr), h->namps, du,
-                       nshots, dout);
+                       nshots, (int)strict, dout);
     QH_CHECK(hipGetLastError());
     QH_CHECK(hipMemcpyAsync(out, dout, nshots * sizeof(uint64_t), hipMemcpyDeviceToHost,
                             h->stream));

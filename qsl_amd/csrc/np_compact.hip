@@ -294,23 +294,24 @@ __global__ __launch_bounds__(kBlock) void k_np_search(const double *__restrict__
                                                       uint64_t namps, NPDev d,
                                                       uint64_t nseg,
                                                       const double *__restrict__ u,
-                                                      uint64_t nshots,
+                                                      uint64_t nshots, int strict,
                                                       uint64_t *__restrict__ out)
 {
     uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nshots) return;
     double uu = u[i];
-    bool zero = !(uu > 0.0);
+    bool zero = strict || !(uu > 0.0);
+    double key = strict ? uu : 0.0;
     uint64_t lo = 0, hi = namps;
     while (lo < hi) {
         uint64_t mid = lo + ((hi - lo) >> 1);
         double v = cdf[mid];
-        bool ge = zero ? (v > 0.0) : (v >= uu);
+        bool ge = zero ? (v > key) : (v >= uu);
         if (ge) hi = mid;
         else lo = mid + 1;
     }
     if (lo >= namps) {
-        out[i] = 0;
+        out[i] = strict ? ~0ull : 0ull;
         return;
     }
     // last segment whose start is <= lo
@@ -542,7 +543,8 @@ void np_gather_full(qslhip_state *h, const void *dev_full)
 namespace qslhip {
 void build_cdf_any(qslhip_state *h);
 
-void np_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out)
+void np_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out,
+               bool strict)
 {
     build_cdf_any(h);
     if (nshots == 0) {
@@ -557,7 +559,7 @@ void np_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out)
     NPDev d = dev_view(h);
     hipLaunchKernelGGL(k_np_search, dim3(grid_for(nshots, kBlock)), dim3(kBlock), 0,
                        h->stream, static_cast<const double *>(h->cdf.ptr), h->namps, d,
-                       (uint64_t)1 << h->npt->hib, du, nshots, dout);
+                       (uint64_t)1 << h->npt->hib, du, nshots, (int)strict, dout);
     QH_CHECK(hipGetLastError());
     QH_CHECK(hipMemcpyAsync(out, dout, nshots * sizeof(uint64_t), hipMemcpyDeviceToHost,
                             h->stream));

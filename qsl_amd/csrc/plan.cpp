@@ -25,6 +25,13 @@ bool is_np_kind(int kind)
     }
 }
 
+// cos and sin of one argument the way the reference's g++ -O2 build gets
+// them: GCC folds the adjacent std::cos(x), std::sin(x) calls of every gate
+// into one glibc sincos(x) call, whose cosine can differ from cos(x) by one
+// ulp.  Calling sincos here keeps the coefficients bit-identical.
+static inline void fp_sincos(double x, double &c, double &s) { ::sincos(x, &s, &c); }
+static inline void fp_sincos(float x, float &c, float &s) { ::sincosf(x, &s, &c); }
+
 // Coefficients exactly as the reference computes them, in Fp:
 //   rotations:  Fp cos = std::cos(angle/2), sin = std::sin(angle/2)
 //               (src/qubits/default/gates.cpp:124-125)
@@ -33,16 +40,19 @@ bool is_np_kind(int kind)
 template <typename Fp>
 static void coeffs(int kind, Fp angle, double &c, double &s)
 {
+    Fp fc, fs;
     switch (kind) {
     case QSLHIP_PHASE: case QSLHIP_CPHASE:
-        c = (double)std::cos(angle);
-        s = (double)std::sin(angle);
+        fp_sincos(angle, fc, fs);
+        c = (double)fc;
+        s = (double)fs;
         break;
     case QSLHIP_ROT_X: case QSLHIP_ROT_Y: case QSLHIP_ROT_Z:
     case QSLHIP_CROT_X: case QSLHIP_CROT_Y: case QSLHIP_CROT_Z:
     case QSLHIP_NP_ROT_X: case QSLHIP_NP_ROT_Y:
-        c = (double)std::cos(angle / 2);
-        s = (double)std::sin(angle / 2);
+        fp_sincos((Fp)(angle / 2), fc, fs);
+        c = (double)fc;
+        s = (double)fs;
         break;
     default:
         c = 0;

@@ -1,0 +1,408 @@
+// test_qubits_hip.cpp -- typed tests of qsl::Qubits<Type::Hip, Fp> and
+// qsl::Qubits<Type::HipNP, Fp> through the C++20 API, in the shape of the
+// reference's gtest suite (src/gtest/sim-constructors.cpp, gates.cpp,
+// measurements.cpp).  The checker is the CPU oracle (oracle/liboracle.so),
+// whose gate arithmetic is pinned bit for bit to the reference's own build.
+//
+//   ./test_qubits_hip            run everything (needs an MI355X)
+//   ./test_qubits_hip --compile  only the compile-time concept checks
+#include <qsl/qubits.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <sstream>
+#include <string>
+#include <vector>
+
+extern "C" {
+#include "qsl_oracle.h"
+}
+
+// ---- compile-time contract (include/qsl/concepts.hpp) ----------------------
+static_assert(qsl::Simulator<qsl::Qubits<qsl::Type::Hip, double>>);
+static_assert(qsl::Simulator<qsl::Qubits<qsl::Type::Hip, float>>);
+static_assert(qsl::HasAllGates<qsl::Qubits<qsl::Type::Hip, double>>);
+static_assert(qsl::NonNPSimulator<qsl::Qubits<qsl::Type::Hip, float>>);
+static_assert(qsl::NPSimulator<qsl::Qubits<qsl::Type::Hip, double>>);
+static_assert(qsl::Simulator<qsl::Qubits<qsl::Type::HipNP, double>>);
+static_assert(qsl::NPSimulator<qsl::Qubits<qsl::Type::HipNP, double>>);
+static_assert(qsl::NPSimulator<qsl::Qubits<qsl::Type::HipNP, float>>);
+static_assert(!qsl::HasNonNPGates<qsl::Qubits<qsl::Type::HipNP, double>>);
+
+// ---- a minimal test runner ---------------------------------------------------
+static int g_fail = 0, g_checks = 0;
+static std::vector<std::pair<std::string, std::function<void()>>> &registry()
+{
+    static std::vector<std::pair<std::string, std::function<void()>>> r;
+    return r;
+}
+struct Reg {
+    Reg(const char *n, std::function<void()> f) { registry().emplace_back(n, f); }
+};
+#define TEST(name)                        \
+    static void name();                   \
+    static Reg reg_##name(#name, name);   \
+    static void name()
+#define EXPECT(cond)                                                              \
+    do {                                                                          \
+        g_checks++;                                                               \
+        if (!(cond)) {                                                            \
+            g_fail++;                                                             \
+            std::fprintf(stderr, "  FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+        }                                                                         \
+    } while (0)
+#define EXPECT_THROW(stmt, exc)              \
+    do {                                     \
+        bool thrown_ = false;                \
+        try {                                \
+            stmt;                            \
+        } catch (const exc &) {              \
+            thrown_ = true;                  \
+        }                                    \
+        EXPECT(thrown_ && #stmt " throws " #exc); \
+    } while (0)
+
+template <typename Fp>
+using Hip = qsl::Qubits<qsl::Type::Hip, Fp>;
+template <typename Fp>
+using HipNP = qsl::Qubits<qsl::Type::HipNP, Fp>;
+
+// oracle dispatch by precision
+template <typename Fp>
+struct Orc;
+template <>
+struct Orc<double> {
+    static void g1(double *s, unsigned n, int k, unsigned t, double a) { orc_gate1_d(s, n, k, t, a); }
+    static void g2(double *s, unsigned n, int k, unsigned a, unsigned b, double x)
+    {
+        orc_gate2_d(s, n, k, a, b, x);
+    }
+    static double prob(const double *s, unsigned n, unsigned t, unsigned o) { return orc_prob_d(s, n, t, o); }
+    static int measure(double *s, unsigned n, unsigned t, double u) { return orc_measure_d(s, n, t, u); }
+    static void sample_all(const double *s, unsigned n, const double *u, size_t m, uint64_t *o)
+    {
+        orc_sample_all_d(s, n, u, m, o);
+    }
+    static void sample_all2(const double *s, unsigned n, const double *u, size_t m, uint64_t *c)
+    {
+        orc_sample_all2_d(s, n, u, m, c);
+    }
+};
+template <>
+struct Orc<float> {
+    static void g1(float *s, unsigned n, int k, unsigned t, float a) { orc_gate1_f(s, n, k, t, a); }
+    static void g2(float *s, unsigned n, int k, unsigned a, unsigned b, float x)
+    {
+        orc_gate2_f(s, n, k, a, b, x);
+    }
+    static float prob(const float *s, unsigned n, unsigned t, unsigned o) { return orc_prob_f(s, n, t, o); }
+    static int measure(float *s, unsigned n, unsigned t, float u) { return orc_measure_f(s, n, t, u); }
+    static void sample_all(const float *s, unsigned n, const float *u, size_t m, uint64_t *o)
+    {
+        orc_sample_all_f(s, n, u, m, o);
+    }
+    static void sample_all2(const float *s, unsigned n, const float *u, size_t m, uint64_t *c)
+    {
+        orc_sample_all2_f(s, n, u, m, c);
+    }
+};
+
+template <typename Fp>
+static Fp *raw(std::vector<qsl::complex<Fp>> &v)
+{
+    return reinterpret_cast<Fp *>(v.data());
+}
+
+template <typename Fp>
+static bool same_bits(const std::vector<qsl::complex<Fp>> &a, const std::vector<qsl::complex<Fp>> &b)
+{
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size() * sizeof(a[0])) == 0;
+}
+
+template <typename Fp>
+static double max_diff(const std::vector<qsl::complex<Fp>> &a, const std::vector<qsl::complex<Fp>> &b)
+{
+    double m = 0;
+    for (size_t i = 0; i < a.size(); i++)
+        m = std::fmax(m, std::fabs((double)a[i].real - b[i].real) +
+                             std::fabs((double)a[i].imag - b[i].imag));
+    return m;
+}
+
+// ---- constructors and utilities (sim-constructors.cpp:107-260) --------------
+template <typename Fp>
+static void constructor_test()
+{
+    using Sim = Hip<Fp>;
+    const unsigned n = 5;
+    Sim q{n};
+    const auto state = qsl::makeRandomState<Fp>(n, 11);
+    q.setState(state);
+    Sim q_copy{q};
+    EXPECT(q.getNumQubits() == q_copy.getNumQubits());
+    EXPECT(same_bits(q.getState(), q_copy.getState()));
+    Sim q_from_state{state};
+    EXPECT(same_bits(q.getState(), q_from_state.getState()));
+    Sim q_new{n};
+    q_new = q;
+    EXPECT(same_bits(q.getState(), q_new.getState()));
+    Sim q_wrong{n + 1};
+    EXPECT_THROW(q_wrong = q, std::logic_error);
+    // utility functions
+    Sim q3{3};
+    EXPECT_THROW(q3.setState(qsl::makeRandomState<Fp>(4, 1)), std::logic_error);
+    q3.setBasisState(4);
+    auto b = q3.getState();
+    EXPECT(b[4].real == 1 && std::fabs(qsl::norm(b) - 1) < 1e-6);
+    EXPECT_THROW(q3.hadamard(3), std::out_of_range);
+    EXPECT(Sim::name == (sizeof(Fp) == 8 ? "Qub<hip,double>" : "Qub<hip,float>"));
+}
+TEST(ConstructorTestDouble) { constructor_test<double>(); }
+TEST(ConstructorTestFloat) { constructor_test<float>(); }
+
+template <typename Fp>
+static void np_constructor_test()
+{
+    using Sim = HipNP<Fp>;
+    const unsigned n = 5;
+    Sim q{n};
+    const auto state = qsl::makeRandomNPState<Fp>(n, 3, 5);
+    q.setState(state);
+    EXPECT(q.getNumOnes() == 3);
+    Sim q_copy{q};
+    EXPECT(same_bits(q.getState(), q_copy.getState()));
+    Sim q_from_state{state};
+    EXPECT(same_bits(q.getState(), q_from_state.getState()));
+    Sim q_new{n};
+    q_new = q;
+    EXPECT(same_bits(q.getState(), q_new.getState()));
+    Sim q_wrong{n + 1};
+    EXPECT_THROW(q_wrong = q, std::logic_error);
+    // sim-constructors.cpp:202-233
+    EXPECT_THROW(Sim(4, 5), std::logic_error);
+    Sim q4{4};
+    EXPECT_THROW(q4.setNumOnes(5), std::logic_error);
+    q4.setNumOnes(2);
+    EXPECT(q4.getNumOnes() == 2);
+    EXPECT_THROW(q4.setState(qsl::makeRandomNPState<Fp>(5, 2, 1)), std::logic_error);
+    q4.setBasisState(4);
+    auto b = q4.getState();
+    EXPECT(b[4].real == 1 && q4.getNumOnes() == 1);
+}
+TEST(NPConstructorTestDouble) { np_constructor_test<double>(); }
+TEST(NPConstructorTestFloat) { np_constructor_test<float>(); }
+
+// print format (sim-constructors.cpp:254-293)
+TEST(PrintTest)
+{
+    Hip<double> q{3};
+    std::stringstream s1, s2;
+    q.print(s1);
+    s2 << "Number of qubits = 3" << std::endl << "(1, 0)" << std::endl;
+    for (int i = 0; i < 7; i++) s2 << "(0, 0)" << std::endl;
+    s2 << std::endl;
+    EXPECT(s1.str() == s2.str());
+
+    HipNP<float> p{3, 2};
+    std::stringstream t1, t2;
+    p.print(t1);
+    t2 << "Number of qubits = 3" << std::endl << "Number of ones = 2" << std::endl;
+    for (int i = 0; i < 3; i++) t2 << "(0, 0)" << std::endl;
+    t2 << "(1, 0)" << std::endl;
+    for (int i = 0; i < 4; i++) t2 << "(0, 0)" << std::endl;
+    t2 << std::endl;
+    EXPECT(t1.str() == t2.str());
+}
+
+// ---- gates against the oracle (gates.cpp:313-878 shape: n = 8, targ 4,
+// ctrl 2, random state, random angle), here over every target pair ---------
+template <typename Fp>
+static void gate_test()
+{
+    const unsigned n = 8;
+    Hip<Fp> q{n};
+    const auto st0 = qsl::makeRandomState<Fp>(n, 42);
+    using G1 = void (Hip<Fp>::*)(unsigned);
+    using G1a = void (Hip<Fp>::*)(unsigned, Fp);
+    using G2 = void (Hip<Fp>::*)(unsigned, unsigned);
+    using G2a = void (Hip<Fp>::*)(unsigned, unsigned, Fp);
+    const std::vector<std::pair<int, G1>> g1 = {
+        {QSLHIP_PAULI_X, &Hip<Fp>::pauliX}, {QSLHIP_PAULI_Y, &Hip<Fp>::pauliY},
+        {QSLHIP_PAULI_Z, &Hip<Fp>::pauliZ}, {QSLHIP_HADAMARD, &Hip<Fp>::hadamard}};
+    const std::vector<std::pair<int, G1a>> g1a = {
+        {QSLHIP_PHASE, &Hip<Fp>::phase}, {QSLHIP_ROT_X, &Hip<Fp>::rotateX},
+        {QSLHIP_ROT_Y, &Hip<Fp>::rotateY}, {QSLHIP_ROT_Z, &Hip<Fp>::rotateZ}};
+    const std::vector<std::pair<int, G2>> g2 = {
+        {QSLHIP_CNOT, &Hip<Fp>::controlNot}, {QSLHIP_CY, &Hip<Fp>::controlY},
+        {QSLHIP_CZ, &Hip<Fp>::controlZ}, {QSLHIP_CHADAMARD, &Hip<Fp>::controlHadamard},
+        {QSLHIP_SWAP, &Hip<Fp>::swap}, {QSLHIP_FSWAP, &Hip<Fp>::fswap},
+        {QSLHIP_NP_HADAMARD, &Hip<Fp>::npHadamard}};
+    const std::vector<std::pair<int, G2a>> g2a = {
+        {QSLHIP_CROT_X, &Hip<Fp>::controlRotateX}, {QSLHIP_CROT_Y, &Hip<Fp>::controlRotateY},
+        {QSLHIP_CROT_Z, &Hip<Fp>::controlRotateZ}, {QSLHIP_CPHASE, &Hip<Fp>::controlPhase},
+        {QSLHIP_NP_ROT_X, &Hip<Fp>::npRotateX}, {QSLHIP_NP_ROT_Y, &Hip<Fp>::npRotateY}};
+    const Fp angle = static_cast<Fp>(M_PI * 0.7316);
+    for (unsigned t = 0; t < n; t++) {
+        for (auto &[k, f] : g1) {
+            q.setState(st0);
+            (q.*f)(t);
+            auto want = st0;
+            Orc<Fp>::g1(raw(want), n, k, t, 0);
+            EXPECT(same_bits(q.getState(), want));
+        }
+        for (auto &[k, f] : g1a) {
+            q.setState(st0);
+            (q.*f)(t, angle);
+            auto want = st0;
+            Orc<Fp>::g1(raw(want), n, k, t, angle);
+            EXPECT(same_bits(q.getState(), want));
+        }
+        for (unsigned c = 0; c < n; c++) {
+            if (c == t) continue;
+            for (auto &[k, f] : g2) {
+                q.setState(st0);
+                (q.*f)(c, t);
+                auto want = st0;
+                Orc<Fp>::g2(raw(want), n, k, c, t, 0);
+                EXPECT(same_bits(q.getState(), want));
+            }
+            for (auto &[k, f] : g2a) {
+                q.setState(st0);
+                (q.*f)(c, t, angle);
+                auto want = st0;
+                Orc<Fp>::g2(raw(want), n, k, c, t, angle);
+                EXPECT(same_bits(q.getState(), want));
+            }
+        }
+    }
+}
+TEST(GatesDouble) { gate_test<double>(); }
+TEST(GatesFloat) { gate_test<float>(); }
+
+// ---- measurement (measurements.cpp:80-856) -------------------------------------
+template <typename Fp>
+static void measurement_test()
+{
+    const unsigned n = 8;
+    const double tol = sizeof(Fp) == 8 ? 1e-12 : 1e-5;
+    auto st = qsl::makeRandomState<Fp>(n, 7);
+    Hip<Fp> q{st};
+    for (unsigned t = 0; t < n; t++)
+        for (unsigned o = 0; o < 2; o++)
+            EXPECT(std::fabs(q.prob(t, o) - Orc<Fp>::prob(raw(st), n, t, o)) < tol);
+    // postselect: prob(targ, outcome) becomes 1 (measurements.cpp:160-283)
+    {
+        Hip<Fp> p{st};
+        Fp pr = p.postselect(3, 1);
+        EXPECT(std::fabs(pr - Orc<Fp>::prob(raw(st), n, 3, 1)) < tol);
+        EXPECT(std::fabs(p.prob(3, 1) - 1) < 10 * tol);
+    }
+    // measure with injected uniforms: identical outcome and collapse
+    for (unsigned t = 0; t < n; t++) {
+        Hip<Fp> m{st};
+        auto want = st;
+        Fp u = static_cast<Fp>(0.37);
+        EXPECT(m.measure(t, u) == Orc<Fp>::measure(raw(want), n, t, u));
+        EXPECT(max_diff(m.getState(), want) < 10 * tol);
+    }
+    // sampleAll / sampleAll2 with injected uniforms
+    std::vector<Fp> u(4000);
+    std::mt19937_64 g(3);
+    for (auto &x : u) x = static_cast<Fp>((g() >> 11) * 0x1.0p-53);
+    auto m1 = q.sampleAll(u);
+    std::vector<uint64_t> o(u.size());
+    Orc<Fp>::sample_all(raw(st), n, u.data(), u.size(), o.data());
+    std::map<std::size_t, std::size_t> m2;
+    for (auto x : o) m2[x]++;
+    EXPECT(m1 == m2);
+    auto a1 = q.sampleAll2(u);
+    std::vector<uint64_t> c(1u << n);
+    Orc<Fp>::sample_all2(raw(st), n, u.data(), u.size(), c.data());
+    bool same = a1.size() == c.size();
+    for (size_t i = 0; same && i < c.size(); i++) same = a1[i] == c[i];
+    EXPECT(same);
+    // statistical check of the self-drawn streams, as the reference does:
+    // 1000 samples, |p_est - p_true| < 0.1 (measurements.cpp:39)
+    q.seed(1234);
+    auto cnt = q.sample(2, 1000);
+    EXPECT(cnt[0] + cnt[1] == 1000);
+    EXPECT(std::fabs(cnt[0] / 1000.0 - q.prob(2, 0)) < 0.1);
+    auto all = q.sampleAll(1000);
+    size_t tot = 0;
+    for (auto &[k, v] : all) tot += v;
+    EXPECT(tot == 1000);
+    // measureAll collapses onto the drawn basis state
+    std::size_t x = q.measureAll();
+    auto s2 = q.getState();
+    EXPECT(s2[x].real == 1 && qsl::norm(s2) == 1);
+}
+TEST(MeasurementDouble) { measurement_test<double>(); }
+TEST(MeasurementFloat) { measurement_test<float>(); }
+
+template <typename Fp>
+static void np_gate_test()
+{
+    const unsigned n = 8, k = 4;
+    auto st0 = qsl::makeRandomNPState<Fp>(n, k, 9);
+    HipNP<Fp> q{st0};
+    const Fp angle = static_cast<Fp>(1.2345);
+    for (unsigned a = 0; a < n; a++) {
+        q.setState(st0);
+        q.rotateZ(a, angle);
+        auto want = st0;
+        if constexpr (sizeof(Fp) == 8) orc_np_gate1_d(raw(want), n, k, QSLHIP_ROT_Z, a, angle);
+        else orc_np_gate1_f(raw(want), n, k, QSLHIP_ROT_Z, a, angle);
+        EXPECT(same_bits(q.getState(), want));
+        for (unsigned b = 0; b < n; b++) {
+            if (a == b) continue;
+            q.setState(st0);
+            q.npRotateX(a, b, angle);
+            q.fswap(a, b);
+            q.controlPhase(a, b, angle);
+            auto w = st0;
+            if constexpr (sizeof(Fp) == 8) {
+                orc_np_gate2_d(raw(w), n, k, QSLHIP_NP_ROT_X, a, b, angle);
+                orc_np_gate2_d(raw(w), n, k, QSLHIP_FSWAP, a, b, 0);
+                orc_np_gate2_d(raw(w), n, k, QSLHIP_CPHASE, a, b, angle);
+            } else {
+                orc_np_gate2_f(raw(w), n, k, QSLHIP_NP_ROT_X, a, b, angle);
+                orc_np_gate2_f(raw(w), n, k, QSLHIP_FSWAP, a, b, 0);
+                orc_np_gate2_f(raw(w), n, k, QSLHIP_CPHASE, a, b, angle);
+            }
+            EXPECT(same_bits(q.getState(), w));
+        }
+    }
+    // the compact vector is the weight-k amplitudes in generateLookup order
+    auto compact = q.getCompactState();
+    auto full = q.getState();
+    auto lst = qsl::generateLookup(n, k);
+    bool ok = compact.size() == lst.size();
+    for (size_t i = 0; ok && i < lst.size(); i++)
+        ok = std::memcmp(&compact[i], &full[lst[i]], sizeof(compact[i])) == 0;
+    EXPECT(ok);
+}
+TEST(NPGatesDouble) { np_gate_test<double>(); }
+TEST(NPGatesFloat) { np_gate_test<float>(); }
+
+int main(int argc, char **argv)
+{
+    if (argc > 1 && std::strcmp(argv[1], "--compile") == 0) {
+        std::printf("compile-time concept checks passed\n");
+        return 0;
+    }
+    for (auto &[name, fn] : registry()) {
+        int before = g_fail;
+        try {
+            fn();
+        } catch (const std::exception &e) {
+            g_fail++;
+            std::fprintf(stderr, "  EXCEPTION in %s: %s\n", name.c_str(), e.what());
+        }
+        std::printf("[%s] %s\n", g_fail == before ? " OK " : "FAIL", name.c_str());
+        std::fflush(stdout);
+    }
+    std::printf("%d checks, %d failures\n", g_checks, g_fail);
+    return g_fail == 0 ? 0 : 1;
+}

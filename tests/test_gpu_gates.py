@@ -113,3 +113,21 @@ def test_fused_equals_per_gate_at_22_qubits():
     O.gate1(want, 3, 0)
     O.apply_ops(want, ops)
     assert np.abs(sa - want).max() <= 1e-12
+
+
+def test_coefficients_follow_the_reference_sincos():
+    # The reference build takes cos/sin of each angle from one glibc sincos
+    # call; a separate cos() differs by an ulp for some angles.  Sweep many
+    # angles through the coefficient path of both precisions.
+    rng = np.random.default_rng(77)
+    for dtype in (np.complex128, np.complex64):
+        st0 = random_state(rng, 2, dtype)
+        q = qsl_amd.Qubits(2, dtype=dtype)
+        for ang in rng.uniform(-20, 20, 400):
+            ang = float(np.float32(ang)) if dtype == np.complex64 else float(ang)
+            for kind in (5, 6, 4):
+                q.setState(st0)
+                q._g1(kind, 1, ang)
+                want = st0.copy()
+                O.gate1(want, kind, 1, ang)
+                assert _bits_equal(q.getState(), want), (dtype, kind, ang)

@@ -1,0 +1,171 @@
+"""GPU parity of the compact number-preserving backend.
+
+Checked against (a) the oracle's NP semantics on the full 2^n vector (the
+reference's NP backend restated, src/qubits/np/*.cpp) at n <= 14, bit for
+bit, and (b) the oracle's compact colex layout (the same amplitude order the
+device uses) at n = 20..24, and (c) size-independent properties at n = 32,
+k = 16 (config 3): norm preservation, a gate sequence followed by its
+inverse returns the exact start state, and rank/unrank round trips via
+set_basis / get_amps.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import qsl_amd
+from oracle.pyoracle import (Oracle, OP_DTYPE, NP_GATE1, NP_GATE2, HAS_ANGLE,
+                             random_np_state)
+
+pytestmark = pytest.mark.gpu
+O = Oracle()
+
+
+def _bits_equal(a, b):
+    return np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def _angle(rng, dtype):
+    a = float(rng.uniform(-2 * np.pi, 2 * np.pi))
+    return float(np.float32(a)) if dtype == np.complex64 else a
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,k", [(4, 2), (7, 3), (10, 5), (13, 4)])
+def test_np_gates_vs_full_vector_semantics(dtype, n, k):
+    rng = np.random.default_rng(n * 100 + k)
+    st0 = random_np_state(rng, n, k, dtype)
+    q = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    for kind in NP_GATE1:
+        for t in range(n):
+            ang = _angle(rng, dtype) if kind in HAS_ANGLE else 0.0
+            q.setState(st0)
+            q._g1(kind, t, ang)
+            want = st0.copy()
+            O.np_gate1(want, k, kind, t, ang)
+            assert _bits_equal(q.getState(), want), (kind, t)
+    pairs = [(a, b) for a in range(n) for b in range(n) if a != b]
+    if len(pairs) > 30:
+        pairs = [pairs[i] for i in rng.choice(len(pairs), 30, replace=False)]
+    for kind in NP_GATE2:
+        for a, b in pairs:
+            ang = _angle(rng, dtype) if kind in HAS_ANGLE else 0.0
+            q.setState(st0)
+            q._g2(kind, a, b, ang)
+            want = st0.copy()
+            O.np_gate2(want, k, kind, a, b, ang)
+            assert _bits_equal(q.getState(), want), (kind, a, b)
+
+
+def _np_ops(rng, n, count):
+    ops = np.zeros(count, OP_DTYPE)
+    kinds = NP_GATE1 + NP_GATE2
+    for i in range(count):
+        kind = int(rng.choice(kinds))
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        ops[i] = (kind, a, b if kind >= 16 else 0, 0, rng.uniform(0, 2 * np.pi))
+    return ops
+
+
+@pytest.mark.parametrize("n,k", [(20, 10), (22, 7), (24, 12)])
+def test_np_compact_vs_compact_oracle(n, k):
+    rng = np.random.default_rng(n + k)
+    total = math.comb(n, k)
+    amp = (rng.normal(size=total) + 1j * rng.normal(size=total)).astype(np.complex128)
+    amp /= np.linalg.norm(amp)
+    q = qsl_amd.QubitsNP(n, k)
+    q.setAmps(amp)
+    ops = _np_ops(rng, n, 40)
+    # adjacent bonds of both kinds of segments (low part, high part, across)
+    for b in (0, 7, 14, 15, 16, n - 2):
+        ops = np.concatenate([ops, np.array([(26, b, b + 1, 0, 0.7), (27, b + 1, b, 0, 1.1)],
+                                            OP_DTYPE)])
+    q.apply_ops(ops)
+    want = amp.copy()
+    O.npc_apply_ops(want, n, k, ops, nthreads=8)
+    got = q.getAmps()
+    assert np.abs(got - want).max() <= 1e-12
+    assert _bits_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_np_measurement(dtype):
+    n, k = 11, 5
+    rng = np.random.default_rng(11)
+    st = random_np_state(rng, n, k, dtype)
+    q = qsl_amd.QubitsNP(st)
+    assert q.getNumOnes() == k
+    tol = 1e-12 if dtype == np.complex128 else 1e-5
+    for t in range(n):
+        for o in (0, 1):
+            assert abs(q.prob(t, o) - O.np_prob(st, k, t, o)) <= tol
+    u = qsl_amd.uniforms(3, 5000)
+    got = q.sampleAllOutcomes(u)
+    want = O.sample_all(st, u.astype(np.float32) if dtype == np.complex64 else u)
+    assert np.mean(got == want) > 0.999
+    assert all(bin(int(x)).count("1") == k for x in got)
+    # measure / postselect against the full-vector oracle
+    q2 = qsl_amd.QubitsNP(st)
+    out = q2.measure(3, 0.4)
+    w = st.copy()
+    assert out == O.measure(w, 3, 0.4)
+    assert np.abs(q2.getState() - w).max() <= 10 * tol
+
+
+def test_np_state_management():
+    q = qsl_amd.QubitsNP(6, 2)
+    st = q.getState()
+    assert np.nonzero(st)[0].tolist() == [3]  # np.cpp:77-78
+    q.setBasisState(0b101100)
+    assert q.getNumOnes() == 3
+    assert np.nonzero(q.getState())[0].tolist() == [0b101100]
+    q.setNumOnes(4)
+    assert np.nonzero(q.getState())[0].tolist() == [0b1111]
+    with pytest.raises(qsl_amd.NotNPError):
+        q.setNumOnes(6)
+    with pytest.raises(qsl_amd.NotNPError):
+        qsl_amd.QubitsNP(4, 5)
+    bad = np.zeros(64, np.complex128)
+    bad[1] = bad[3] = 1 / np.sqrt(2)
+    with pytest.raises(qsl_amd.NotNPError):
+        q.setState(bad)
+    with pytest.raises(qsl_amd.SizeError):
+        q.setState(np.zeros(32, np.complex128))
+    with pytest.raises(qsl_amd.QslHipError):
+        q.hadamard(0)  # not number preserving
+
+
+def test_np_n32_k16_properties():
+    n, k = 32, 16
+    q = qsl_amd.QubitsNP(n, k)
+    total = math.comb(n, k)
+    assert q.numAmps() == total
+    rng = np.random.default_rng(32)
+    # rank/unrank: set a random basis string, read back its compact index
+    for _ in range(5):
+        bits = rng.choice(n, k, replace=False)
+        x = int(sum(1 << int(b) for b in bits))
+        q.setBasisState(x)
+        r = O.rank(x)
+        assert q.getAmps(r, 1)[0] == 1
+        assert abs(q.norm() - 1) < 1e-15
+        out = q.sampleAllOutcomes(np.array([0.5]))
+        assert int(out[0]) == x
+    # forward layers then exact inverses return to the start string
+    x0 = O.unrank(total // 3, k)
+    q.setBasisState(x0)
+    ops = []
+    for layer in range(2):
+        for b in range(layer % 2, n - 1, 2):
+            ops.append((26 + (b % 2), b, b + 1, 0, 0.3 + 0.01 * b))
+        for t in range(n):
+            ops.append((4, t, 0, 0, 0.1 * t))
+    ops = np.array(ops, OP_DTYPE)
+    q.apply_ops(ops)
+    assert abs(q.norm() - 1) < 1e-12
+    inv = ops[::-1].copy()
+    inv["angle"] = -inv["angle"]
+    q.apply_ops(inv)
+    amp = q.getAmps(O.rank(x0), 1)[0]
+    assert abs(amp - 1) < 1e-12
+    assert abs(q.norm() - 1) < 1e-12

@@ -115,7 +115,7 @@ def test_np_reset_index():
     # NP reset puts the amplitude at 2^nones - 1 (sim-constructors.cpp:272-293:
     # n=3, nones=2 prints (1,0) on line 4)
     st = np.zeros(8, np.complex128)
-    O.np_reset(st, 3, 2)
+    O.np_reset(st, 2)
     assert np.nonzero(st)[0].tolist() == [3]
     assert O.rank(3) == 0
 
