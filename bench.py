@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["fused", "per-gate"], default="fused")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short fixed run for rocprofv3 counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus "
@@ -161,6 +163,26 @@ def main():
 
     mode = qsl_amd.MODE_FUSED if args.mode == "fused" else qsl_amd.MODE_PER_GATE
     ops, stream = W.c2_random(N, NGATES)
+    if args.profile:
+        # a short fixed run for rocprofv3 counter passes: the fused circuit
+        # once (15 tile passes), rotateX on a low and a high qubit, and one
+        # device-to-device copy of the 16 GiB state (2^30*16 B read and
+        # written: the calibration for FETCH_SIZE / WRITE_SIZE)
+        q = qsl_amd.Qubits(N, dtype=np.complex128)
+        print("profile: fused circuit", flush=True)
+        q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+        q.sync()
+        print("profile: rotateX t=5 x3, t=29 x3", flush=True)
+        for t in (5, 29):
+            for _ in range(3):
+                q.rotateX(t, 0.3)
+        q.sync()
+        print("profile: state copy", flush=True)
+        c = q.copy()
+        c.sync()
+        del c
+        print("profile: done", flush=True)
+        return
     shots_u = stream.take(NSHOTS)
     masks, counts = L.plan(N, 8, ops)
     npasses = len(masks)

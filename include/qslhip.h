@@ -178,6 +178,49 @@ int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
                   uint64_t *counts);
 int qslhip_norm(qslhip_t h, double *norm);
 
+/* ---- expectation values and adjoint gradients (no reference counterpart;
+ * the reference has no observables or gradients) ----------------------------
+ * An observable is a real combination of Pauli strings.  Qubit q of a term
+ * is X if only bit q of xmask is set, Z if only bit q of zmask, Y if both:
+ *   P|j> = i^nY (-1)^popcount(j & zmask) |j ^ xmask>. */
+typedef struct {
+    double coef;
+    uint64_t xmask;
+    uint64_t zmask;
+} qslhip_pauli;
+
+/* <psi|H|psi> of the handle's state, accumulated in FP64 (full-vector
+ * handles). */
+int qslhip_expval(qslhip_t h, const qslhip_pauli *terms, uint64_t nterms,
+                  double *value);
+
+/* Applies `ops` to the handle's state psi0 (the handle keeps psi = U psi0)
+ * and returns E = <psi|H|psi> and dE/d(angle) of every op (0 for gates
+ * without an angle) by the adjoint reverse sweep: one kernel per gate, in
+ * the handle's precision, FP64 accumulation.  Differentiable kinds: rotateX/
+ * Y/Z, phase, controlRotateX/Y/Z, controlPhase, npRotateX/Y. */
+int qslhip_expval_grad(qslhip_t h, const qslhip_op *ops, uint64_t nops,
+                       const qslhip_pauli *terms, uint64_t nterms,
+                       double *energy, double *grads);
+
+/* Batched variational circuits: ncirc circuits of nqubits[c] qubits, each
+ * started from |0...0>, circuit c owning ops[op_offsets[c] ..
+ * op_offsets[c+1]) and observable terms[term_offsets[c] ..
+ * term_offsets[c+1]).  FP32 states, FP64 energies and gradients, one
+ * persistent kernel captured in a hipGraph at creation.
+ * qslhip_batch_run replays it with new angles (one per op, NULL keeps the
+ * previous ones) and writes energies[ncirc] and grads[total ops]. */
+typedef struct qslhip_batch *qslhip_batch_t;
+int qslhip_batch_create(int prec, uint32_t ncirc, const uint32_t *nqubits,
+                        const uint64_t *op_offsets, const qslhip_op *ops,
+                        const uint64_t *term_offsets, const qslhip_pauli *terms,
+                        qslhip_batch_t *out);
+int qslhip_batch_run(qslhip_batch_t b, const double *angles, double *energies,
+                     double *grads);
+/* The stream the batch graph is launched on (for event timing). */
+int qslhip_batch_stream(qslhip_batch_t b, void **stream);
+int qslhip_batch_destroy(qslhip_batch_t b);
+
 /* ---- synthetic workloads and seeded uniforms (no reference counterpart;
  * the reference's generators are unseedable, src/utils/random.cpp:32-45) --
  * qslhip_uniforms: mt19937_64 seeded with splitmix64(seed), u =

@@ -84,7 +84,15 @@ SIGNATURES = {
     "qslhip_sample_all2": (_I, [_vp, _vp, _u64, _vp, _u64]),
     "qslhip_norm": (_I, [_vp, C.POINTER(_D)]),
     "qslhip_uniforms": (_I, [_u64, _u64, _vp]),
+    "qslhip_expval": (_I, [_vp, _vp, _u64, C.POINTER(_D)]),
+    "qslhip_expval_grad": (_I, [_vp, _vp, _u64, _vp, _u64, C.POINTER(_D), _vp]),
+    "qslhip_batch_create": (_I, [_I, C.c_uint32, _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "qslhip_batch_run": (_I, [_vp, _vp, _vp, _vp]),
+    "qslhip_batch_stream": (_I, [_vp, C.POINTER(_vp)]),
+    "qslhip_batch_destroy": (_I, [_vp]),
 }
+
+PAULI_DTYPE = np.dtype([("coef", "<f8"), ("xmask", "<u8"), ("zmask", "<u8")])
 
 
 def lib():

@@ -134,6 +134,8 @@ void check_np_host(const void *host, uint64_t count, unsigned *nones)
 
 namespace qslhip {
 
+int guarded_call(const std::function<void()> &f) { return guarded(f); }
+
 void DevBuf::ensure(size_t b)
 {
     if (b <= bytes) return;
@@ -751,6 +753,48 @@ int qslhip_norm(qslhip_t h, double *norm)
         QH_REQUIRE(norm, QSLHIP_EINVAL, "null output");
         double s = h->np ? np_prob(h, 0, 0) + np_prob(h, 0, 1) : device_norm2(h);
         *norm = std::sqrt(s);
+    });
+}
+
+static void validate_terms(qslhip_t h, const qslhip_pauli *terms, uint64_t nterms)
+{
+    QH_REQUIRE(terms || nterms == 0, QSLHIP_EINVAL, "null terms");
+    const uint64_t lim = h->n >= 64 ? ~0ull : (1ull << h->n);
+    for (uint64_t t = 0; t < nterms; t++)
+        QH_REQUIRE(terms[t].xmask < lim && terms[t].zmask < lim, QSLHIP_EINVAL,
+                   "Pauli term on a qubit outside the register");
+}
+
+int qslhip_expval(qslhip_t h, const qslhip_pauli *terms, uint64_t nterms, double *value)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(value, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(!h->np, QSLHIP_EKIND, "expectation values need a full-vector handle");
+        validate_terms(h, terms, nterms);
+        *value = expval(h, terms, nterms);
+    });
+}
+
+int qslhip_expval_grad(qslhip_t h, const qslhip_op *ops, uint64_t nops,
+                       const qslhip_pauli *terms, uint64_t nterms, double *energy,
+                       double *grads)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(energy, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
+        QH_REQUIRE(!h->np, QSLHIP_EKIND, "gradients need a full-vector handle");
+        validate_terms(h, terms, nterms);
+        std::vector<GateDesc> gates;
+        gates.reserve(nops);
+        for (uint64_t i = 0; i < nops; i++) {
+            const qslhip_op &o = ops[i];
+            validate_gate(h, o.kind, o.q0, o.q1);
+            gates.push_back(make_desc(h->prec, o.kind, o.q0,
+                                      is_gate2(o.kind) ? o.q1 : 0, o.angle));
+        }
+        adjoint_plain(h, gates, terms, nterms, energy, grads);
     });
 }
 

@@ -107,6 +107,8 @@ __device__ __forceinline__ void rz_half(bool one, cplx<T> &x, T c, T s)
     else      { x.re = r * c - i * s; x.im = i * c + r * s; }
 }
 
+__host__ __device__ constexpr bool is_gate2_dev(int kind) { return kind >= 16 && kind <= 28; }
+
 // Insert a zero bit at position p.
 __device__ __forceinline__ uint64_t ins0(uint64_t x, unsigned p)
 {

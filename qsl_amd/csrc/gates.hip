@@ -144,9 +144,9 @@ __global__ __launch_bounds__(kBlock) void k_fill_basis(cplx<T> *__restrict__ st,
 }
 
 template <typename T>
-void dispatch(qslhip_state *h, const GateDesc &g)
+void dispatch(qslhip_state *h, const GateDesc &g, void *amps)
 {
-    cplx<T> *st = static_cast<cplx<T> *>(h->amps);
+    cplx<T> *st = static_cast<cplx<T> *>(amps);
     const T c = (T)g.c, s = (T)g.s;
     const unsigned n = h->n;
     const unsigned per = kBlock * kItems;
@@ -199,8 +199,9 @@ void dispatch(qslhip_state *h, const GateDesc &g)
 }  // namespace
 
 // Low targets go through a one-op LDS tile; the rest stream.
-void launch_gate(qslhip_state *h, const GateDesc &g)
+void launch_gate(qslhip_state *h, const GateDesc &g, void *amps)
 {
+    if (!amps) amps = h->amps;
     const Tuning &tu = tuning();
     const int lim = h->prec == 8 ? tu.lds_max_target_f64 : tu.lds_max_target_f32;
     const int logt = h->prec == 8 ? tu.logt_f64 : tu.logt_f32;
@@ -211,12 +212,12 @@ void launch_gate(qslhip_state *h, const GateDesc &g)
         std::vector<TilePass> passes;
         std::vector<TileOp> ops;
         plan_tile_passes(h->n, h->prec, one, logt, logt, passes, ops);
-        launch_tile_passes(h, passes, ops);
+        launch_tile_passes(h, passes, ops, amps);
         return;
     }
     if (h->n < 2 && is_gate2(g.kind)) throw Error(QSLHIP_EINVAL, "need 2 qubits");
-    if (h->prec == 8) dispatch<double>(h, g);
-    else dispatch<float>(h, g);
+    if (h->prec == 8) dispatch<double>(h, g, amps);
+    else dispatch<float>(h, g, amps);
 }
 
 void launch_fill_basis(qslhip_state *h, uint64_t index)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -114,9 +115,10 @@ struct qslhip_state {
 namespace qslhip {
 
 // ---- launchers (defined in the .hip files) -----------------------------------
-void launch_gate(qslhip_state *h, const GateDesc &g);
+// amps: apply to this buffer (same size and precision) instead of h->amps
+void launch_gate(qslhip_state *h, const GateDesc &g, void *amps = nullptr);
 void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                        const std::vector<TileOp> &ops);
+                        const std::vector<TileOp> &ops, void *amps = nullptr);
 void launch_fill_basis(qslhip_state *h, uint64_t index);
 double device_prob(qslhip_state *h, unsigned t, unsigned outcome);
 double device_norm2(qslhip_state *h);
@@ -136,6 +138,13 @@ void np_sample(qslhip_state *h, const double *u, uint64_t nshots,
 void np_scatter_full(qslhip_state *h, void *dev_full);
 void np_gather_full(qslhip_state *h, const void *dev_full);
 uint64_t np_rank_host(uint64_t x);
+
+// expectation values and adjoint gradients (adjoint.hip)
+double expval(qslhip_state *h, const qslhip_pauli *terms, uint64_t nterms);
+void adjoint_plain(qslhip_state *h, const std::vector<GateDesc> &gates,
+                   const qslhip_pauli *terms, uint64_t nterms, double *energy,
+                   double *grads);
+int guarded_call(const std::function<void()> &f);
 
 // planner (plan.cpp)
 void plan_tile_passes(unsigned n, int prec, const std::vector<GateDesc> &gates,

@@ -170,6 +170,25 @@ class Qubits:
         ops = np.ascontiguousarray(ops, dtype=L.OP_DTYPE)
         check(L.lib().qslhip_apply_ops(self._h, ptr(ops), len(ops), mode))
 
+    # -- observables and gradients (no reference counterpart) -------------------------
+    def expval(self, terms):
+        """<psi|H|psi> for H = sum of Pauli terms (PAULI_DTYPE records)."""
+        terms = np.ascontiguousarray(terms, dtype=L.PAULI_DTYPE)
+        v = C.c_double()
+        check(L.lib().qslhip_expval(self._h, ptr(terms), len(terms), C.byref(v)))
+        return v.value
+
+    def expvalGrad(self, ops, terms):
+        """Apply ops (the object keeps the result) and return
+        (E, dE/dangle per op) by the adjoint reverse sweep."""
+        ops = np.ascontiguousarray(ops, dtype=L.OP_DTYPE)
+        terms = np.ascontiguousarray(terms, dtype=L.PAULI_DTYPE)
+        e = C.c_double()
+        g = np.zeros(len(ops), np.float64)
+        check(L.lib().qslhip_expval_grad(self._h, ptr(ops), len(ops), ptr(terms),
+                                         len(terms), C.byref(e), ptr(g)))
+        return e.value, g
+
     # -- measurement -------------------------------------------------------------------
     def prob(self, t, outcome):
         p = C.c_double()
@@ -254,3 +273,45 @@ def _fmt(x):
     digits, %g-like, no trailing zeros)."""
     s = "%g" % float(x)
     return s
+
+
+class Batch:
+    """Many variational circuits, one persistent FP32 kernel in a hipGraph
+    (qslhip_batch_*): circuits = list of (nqubits, ops, terms)."""
+
+    def __init__(self, circuits):
+        nq = np.array([c[0] for c in circuits], np.uint32)
+        ops = [np.ascontiguousarray(c[1], dtype=L.OP_DTYPE) for c in circuits]
+        terms = [np.ascontiguousarray(c[2], dtype=L.PAULI_DTYPE) for c in circuits]
+        self.op_off = np.zeros(len(circuits) + 1, np.uint64)
+        self.op_off[1:] = np.cumsum([len(o) for o in ops])
+        t_off = np.zeros(len(circuits) + 1, np.uint64)
+        t_off[1:] = np.cumsum([len(t) for t in terms])
+        self.ops = np.concatenate(ops) if ops else np.zeros(0, L.OP_DTYPE)
+        allt = np.concatenate(terms) if terms else np.zeros(0, L.PAULI_DTYPE)
+        self.ncirc = len(circuits)
+        self._b = None
+        b = _vp()
+        check(L.lib().qslhip_batch_create(4, self.ncirc, ptr(nq), ptr(self.op_off),
+                                          ptr(self.ops), ptr(t_off), ptr(allt),
+                                          C.byref(b)))
+        self._b = b
+
+    def __del__(self):
+        if getattr(self, "_b", None):
+            L.lib().qslhip_batch_destroy(self._b)
+            self._b = None
+
+    def run(self, angles=None):
+        """Energies per circuit and gradients per op (split per circuit)."""
+        e = np.zeros(self.ncirc, np.float64)
+        g = np.zeros(len(self.ops), np.float64)
+        a = None if angles is None else np.ascontiguousarray(angles, np.float64)
+        check(L.lib().qslhip_batch_run(self._b, None if a is None else ptr(a), ptr(e),
+                                       ptr(g)))
+        return e, g
+
+    def stream(self):
+        s = _vp()
+        check(L.lib().qslhip_batch_stream(self._b, C.byref(s)))
+        return s

@@ -119,6 +119,39 @@ def c3_np_layers(n=32, layers=100, stream=None):
     return np.array(ops, OP_DTYPE), r0, s
 
 
+def c4_ansatz(n, layers=8, stream=None):
+    """Hardware-efficient ansatz of config 4: per layer RY(q) for every
+    qubit, RZ(q) for every qubit (angles pi*(2u-1)), then the CNOT ladder
+    0->1->...->n-1."""
+    s = stream or Stream(4)
+    ops = []
+    for _ in range(layers):
+        for q in range(n):
+            ops.append(_op(L.ROT_Y, q, 0, math.pi * (2 * s.take() - 1)))
+        for q in range(n):
+            ops.append(_op(L.ROT_Z, q, 0, math.pi * (2 * s.take() - 1)))
+        for q in range(n - 1):
+            ops.append(_op(L.CNOT, q, q + 1))
+    return np.array(ops, OP_DTYPE), s
+
+
+def c4_observable(n):
+    """H = Z0 Z1 + 0.5 * sum_i X_i as Pauli terms."""
+    t = [(1.0, 0, 0b11)] + [(0.5, 1 << i, 0) for i in range(n)]
+    return np.array(t, L.PAULI_DTYPE)
+
+
+def c4_batch(ncirc=4096, layers=8, stream=None):
+    """Config 4: ncirc circuits with n_c = 10 + floor(11 u) qubits."""
+    s = stream or Stream(4)
+    circuits = []
+    for _ in range(ncirc):
+        n = 10 + min(10, int(11 * s.take()))
+        ops, _ = c4_ansatz(n, layers, s)
+        circuits.append((n, ops, c4_observable(n)))
+    return circuits
+
+
 def c5_haar_layers(n, layers=30, stream=None):
     """Config 5: per layer, RZ(a) RY(b) RZ(g) on every qubit with a, g =
     2*pi*u and b = acos(1 - 2u) (Haar on SU(2)), then controlPhase on a
