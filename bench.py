@@ -141,6 +141,180 @@ def read_traffic():
     return None
 
 
+def bench_c1():
+    """Config 1: 16-qubit FP64, 20 layers (480 gates), prob of every qubit,
+    10^4 shots -- against the reference's serial Default backend."""
+    import qsl_amd
+    from qsl_amd import workloads as W
+    ops, s = W.c1_layers(16, 20)
+    shots = s.take(10 ** 4)
+    q = qsl_amd.Qubits(16)
+    ev = HipEvents()
+    st = stream_of(q)
+    e0, e1 = ev.event(), ev.event()
+    for _ in range(2):
+        q.reset()
+        ev.record(e0, st)
+        q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+        ev.record(e1, st)
+        probs = [q.prob(t, o) for t in range(16) for o in (0, 1)]
+        out = q.sampleAllOutcomes(shots)
+    gates_ms = ev.elapsed_ms(e0, e1)
+    t0 = time.perf_counter()
+    q.reset()
+    q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    probs = [q.prob(t, o) for t in range(16) for o in (0, 1)]
+    out = q.sampleAllOutcomes(shots)
+    wall = time.perf_counter() - t0
+    res = {"gates": len(ops), "gpu_gate_ms": round(gates_ms, 4),
+           "gpu_wall_ms_gates_probs_shots": round(1000 * wall, 3)}
+    try:  # the reference's Default backend, serial, and parity
+        from oracle import pyoracle as P
+        ref = P.RefLib()
+        sim = P.RefSim(ref, P.RefLib.DEFAULT, np.complex128, 16)
+        t0 = time.perf_counter()
+        for op in ops:
+            k = int(op["kind"])
+            if k < 16:
+                sim.gate1(k, int(op["q0"]), float(op["angle"]))
+            else:
+                sim.gate2(k, int(op["q0"]), int(op["q1"]), float(op["angle"]))
+        res["cpu_reference_default_ms"] = round(1000 * (time.perf_counter() - t0), 2)
+        cpu = sim.get_state()
+        q.reset()
+        q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+        gpu = q.getState()
+        res["bit_exact_vs_reference"] = bool(np.array_equal(cpu.view(np.uint8),
+                                                            gpu.view(np.uint8)))
+        o = P.Oracle()
+        want = o.sample_all(cpu, shots)
+        res["shots_equal_to_oracle"] = int(np.sum(want == out))
+    except (FileNotFoundError, OSError) as e:
+        res["cpu_reference"] = f"unavailable: {e}"
+    return res
+
+
+def bench_c3(layers=10):
+    """Config 3: number-preserving 32 qubits, 16 ones, FP64 compact
+    (9.62 GB): npRotateX/Y on alternating bonds + phase on every qubit."""
+    import qsl_amd
+    from qsl_amd import workloads as W
+    from oracle import pyoracle as P
+    n, k = 32, 16
+    ops, r0, _ = W.c3_np_layers(n, layers)
+    q = qsl_amd.QubitsNP(n, k)
+    x0 = P.Oracle().unrank(r0, k)
+    q.setBasisState(x0)
+    ev = HipEvents()
+    st = stream_of(q)
+    e0, e1 = ev.event(), ev.event()
+    q.apply_ops(ops[: 47])  # warm
+    q.setBasisState(x0)
+    q.sync()
+    ev.record(e0, st)
+    q.apply_ops(ops)
+    ev.record(e1, st)
+    ms = ev.elapsed_ms(e0, e1)
+    namps = math.comb(n, k)
+    nb = sum(1 for o in ops if int(o["kind"]) in (26, 27))
+    nph = len(ops) - nb
+    alg = (nb * 2 * 2 * math.comb(n - 2, k - 1) + nph * 2 * math.comb(n - 1, k - 1)) * 16
+    norm = q.norm()
+    return {"layers": layers, "gates": len(ops), "ms": round(ms, 2),
+            "gates_per_s": round(len(ops) / (ms * 1e-3), 1),
+            "algorithmic_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "amplitudes": namps, "state_GB": round(namps * 16 / 1e9, 2),
+            "norm_err": abs(norm - 1)}
+
+
+def bench_c4(steps=3, plain_sample=64):
+    """Config 4: 4096 variational circuits (10-20 qubits), energies and
+    adjoint gradients: batched FP32 (one hipGraph-captured persistent
+    kernel) against the plain per-circuit FP64 GPU path on a stratified
+    sample of the same circuits."""
+    import qsl_amd
+    from qsl_amd import workloads as W
+    circuits = W.c4_batch(4096)
+    t0 = time.perf_counter()
+    b = qsl_amd.Batch(circuits)
+    create_s = time.perf_counter() - t0
+    e, g = b.run()
+    b_stream = b.stream()
+    ev = HipEvents()
+    e0, e1 = ev.event(), ev.event()
+    t0 = time.perf_counter()
+    ev.record(e0, b_stream)
+    for _ in range(steps):
+        e, g = b.run()
+    ev.record(e1, b_stream)
+    wall = (time.perf_counter() - t0) / steps
+    gpu_ms = ev.elapsed_ms(e0, e1) / steps
+    nparams = int(sum(16 * c[0] for c in circuits))
+    # plain FP64 per-circuit path on every (4096 / plain_sample)-th circuit
+    idx = list(range(0, len(circuits), len(circuits) // plain_sample))
+    off = np.concatenate([[0], np.cumsum([len(c[1]) for c in circuits])])
+    maxerr, maxe = 0.0, 0.0
+    t0 = time.perf_counter()
+    for i in idx:
+        n, ops, terms = circuits[i]
+        e64, g64 = qsl_amd.Qubits(n).expvalGrad(ops, terms)
+        maxerr = max(maxerr, float(np.abs(g[off[i]:off[i + 1]] - g64).max()))
+        maxe = max(maxe, abs(float(e[i]) - e64))
+    plain_s = (time.perf_counter() - t0) / len(idx)
+    batch_gps = len(circuits) / wall
+    plain_gps = 1.0 / plain_s
+    return {"circuits": len(circuits), "params": nparams, "batch_create_s": round(create_s, 2),
+            "batch_ms_per_run": round(1000 * wall, 2), "batch_gpu_ms": round(gpu_ms, 2),
+            "gradients_per_s_fp32_batched": round(batch_gps, 1),
+            "params_per_s_fp32_batched": round(nparams / wall, 1),
+            "gradients_per_s_fp64_plain": round(plain_gps, 2),
+            "speedup_vs_fp64_plain": round(batch_gps / plain_gps, 1),
+            "plain_sample": f"{len(idx)} circuits, every {len(circuits) // plain_sample}th, "
+                            "one qslhip_expval_grad call each (FP64, one kernel per gate)",
+            "max_grad_err_vs_fp64": maxerr, "max_energy_err_vs_fp64": maxe}
+
+
+def bench_c5(layers34=6, layers30=30):
+    """Config 5: 34-qubit FP32 (128 GiB) Haar layers fused into dense
+    5-qubit unitaries on matrix cores; error of the same family at 30
+    qubits against the FP64 path (bit-identical to the oracle)."""
+    import qsl_amd
+    from qsl_amd import workloads as W
+    from qsl_amd import _lib as L
+    F = L.tuning_get("dense_f")
+    res = {"fusion_width": F}
+    n = 34
+    ops, _ = W.c5_haar_layers(n, layers34)
+    q = qsl_amd.Qubits(n, dtype=np.complex64)
+    ev = HipEvents()
+    st = stream_of(q)
+    e0, e1 = ev.event(), ev.event()
+    q.apply_ops(ops[: 3 * n], mode=qsl_amd.MODE_DENSE)  # warm
+    q.reset()
+    q.sync()
+    ev.record(e0, st)
+    q.apply_ops(ops, mode=qsl_amd.MODE_DENSE)
+    ev.record(e1, st)
+    ms = ev.elapsed_ms(e0, e1)
+    res.update({"nqubits": n, "layers": layers34, "gates": len(ops), "ms": round(ms, 1),
+                "gates_per_s": round(len(ops) / (ms * 1e-3), 1),
+                "norm_err": abs(q.norm() - 1)})
+    del q
+    n = 30
+    ops, _ = W.c5_haar_layers(n, layers30)
+    q32 = qsl_amd.Qubits(n, dtype=np.complex64)
+    q32.apply_ops(ops, mode=qsl_amd.MODE_DENSE)
+    s32 = q32.getState()
+    del q32
+    q64 = qsl_amd.Qubits(n)
+    q64.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    s64 = q64.getState()
+    del q64
+    res["max_amp_err_30q_vs_fp64"] = float(np.abs(s32.astype(np.complex128) - s64).max())
+    res["max_amp_30q"] = float(np.abs(s64).max())
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +324,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for rocprofv3 counter passes")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline 30-qubit workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus "
@@ -312,6 +488,18 @@ def main():
         result["cpu_baseline"] = cpu_baseline(ops, gpu_state)
     else:
         result["cpu_baseline"] = None
+    del q
+    if not args.no_extras:
+        for name, fn in (("c1_16q_fp64", bench_c1), ("c4_batched_adjoint", bench_c4),
+                         ("c3_np_32q_fp64", bench_c3), ("c5_dense_mfma", bench_c5)):
+            t0 = time.perf_counter()
+            try:
+                result[name] = fn()
+            except Exception as e:  # report, keep the headline line intact
+                result[name] = {"error": f"{type(e).__name__}: {e}"}
+            result[name]["wall_s"] = round(time.perf_counter() - t0, 2)
+            print(f"bench: {name} done in {result[name]['wall_s']} s", file=sys.stderr,
+                  flush=True)
     print(json.dumps(result), flush=True)
 
 

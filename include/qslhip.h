@@ -64,8 +64,11 @@ typedef struct {
 /* Execution modes of qslhip_apply_ops. */
 enum {
     QSLHIP_MODE_PER_GATE = 0, /* one kernel per gate (the reference's loop)  */
-    QSLHIP_MODE_FUSED = 1     /* gates packed into LDS-tile passes; same
+    QSLHIP_MODE_FUSED = 1,    /* gates packed into LDS-tile passes; same
                                  per-amplitude arithmetic, bit-identical     */
+    QSLHIP_MODE_DENSE = 2     /* FP32 handles: gate runs fused into dense
+                                 4/5-qubit unitaries (FP64 products rounded
+                                 once) applied on matrix cores (MFMA)        */
 };
 
 typedef struct qslhip_state *qslhip_t;
@@ -146,6 +149,27 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops,
 int qslhip_plan(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
                 uint64_t *npasses, uint64_t *tile_masks, uint64_t *op_counts,
                 uint64_t cap);
+
+/* ---- tuning (no reference counterpart) ------------------------------------
+ * Keys: "logt_f64", "low_f64", "logt_f32", "low_f32" (LDS tile width and the
+ * contiguous low run of fused passes), "lds_max_target_f64/_f32" (highest
+ * target the per-gate path routes through a one-op tile; -1 = none),
+ * "dense_f" (qubits per fused dense unitary, 4 or 5).  qslhip_autotune times
+ * the candidates on the device (scratch state of min(nqubits, 26) qubits),
+ * keeps the fastest in the process-wide table and writes a JSON report. */
+int qslhip_tuning_get(const char *key, int *value);
+int qslhip_tuning_set(const char *key, int value);
+int qslhip_autotune(int prec, unsigned nqubits, char *report, size_t cap);
+
+/* Dense-plan introspection (host only): how QSLHIP_MODE_DENSE with fusion
+ * width F would split `ops`.  pass_info: 3 u64 per pass (tile mask, first
+ * op, op count); op_info: 9 i64 per op (kind 0 = the single gate gidx as a
+ * tile op, 1 = dense unitary starting at gate gidx; nq; float offset of its
+ * row-major (re, im) matrix in `mats`; its nq qubits, low bit first). */
+int qslhip_plan_dense(unsigned nqubits, const qslhip_op *ops, uint64_t nops, int F,
+                      uint64_t *npasses, uint64_t *nops_out, uint64_t *nfloats,
+                      uint64_t *pass_info, int64_t *op_info, float *mats,
+                      uint64_t cap_passes, uint64_t cap_ops, uint64_t cap_floats);
 
 /* ---- measurement ---------------------------------------------------------------
  * Uniforms are injected (the reference draws them from an unseedable

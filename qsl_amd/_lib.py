@@ -21,7 +21,7 @@ PAULI_X, PAULI_Y, PAULI_Z, HADAMARD, PHASE, ROT_X, ROT_Y, ROT_Z = range(8)
 CNOT, CY, CZ, CROT_X, CROT_Y, CROT_Z, CPHASE, CHADAMARD = range(16, 24)
 SWAP, FSWAP, NP_ROT_X, NP_ROT_Y, NP_HADAMARD = range(24, 29)
 
-MODE_PER_GATE, MODE_FUSED = 0, 1
+MODE_PER_GATE, MODE_FUSED, MODE_DENSE = 0, 1, 2
 
 OP_DTYPE = np.dtype([("kind", "<i4"), ("q0", "<u4"), ("q1", "<u4"),
                      ("pad", "<u4"), ("angle", "<f8")])
@@ -90,6 +90,11 @@ SIGNATURES = {
     "qslhip_batch_run": (_I, [_vp, _vp, _vp, _vp]),
     "qslhip_batch_stream": (_I, [_vp, C.POINTER(_vp)]),
     "qslhip_batch_destroy": (_I, [_vp]),
+    "qslhip_tuning_get": (_I, [C.c_char_p, C.POINTER(_I)]),
+    "qslhip_tuning_set": (_I, [C.c_char_p, _I]),
+    "qslhip_autotune": (_I, [_I, _U, C.c_char_p, C.c_size_t]),
+    "qslhip_plan_dense": (_I, [_U, _vp, _u64, _I, C.POINTER(_u64), C.POINTER(_u64),
+                               C.POINTER(_u64), _vp, _vp, _vp, _u64, _u64, _u64]),
 }
 
 PAULI_DTYPE = np.dtype([("coef", "<f8"), ("xmask", "<u8"), ("zmask", "<u8")])
@@ -150,3 +155,36 @@ def plan(nqubits, prec, ops, cap=4096):
                             ptr(masks), ptr(counts), cap))
     m = min(cap, npasses.value)
     return masks[:m].copy(), counts[:m].copy()
+
+
+def tuning_get(key):
+    v = _I(0)
+    check(lib().qslhip_tuning_get(key.encode(), C.byref(v)))
+    return v.value
+
+
+def tuning_set(key, value):
+    check(lib().qslhip_tuning_set(key.encode(), int(value)))
+
+
+def autotune(prec, nqubits):
+    """Time tile / fusion widths on the device; returns the JSON report."""
+    import json
+    buf = C.create_string_buffer(1 << 14)
+    check(lib().qslhip_autotune(prec, nqubits, buf, len(buf)))
+    return json.loads(buf.value.decode())
+
+
+def plan_dense(nqubits, ops, F):
+    """Dense-mode plan: (pass_info [P,3], op_info [M,9], mats float32)."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    a, b, c = _u64(0), _u64(0), _u64(0)
+    check(lib().qslhip_plan_dense(nqubits, ptr(ops), len(ops), F, C.byref(a), C.byref(b),
+                                  C.byref(c), None, None, None, 0, 0, 0))
+    pi = np.zeros((a.value, 3), np.uint64)
+    oi = np.zeros((b.value, 9), np.int64)
+    m = np.zeros(max(1, c.value), np.float32)
+    check(lib().qslhip_plan_dense(nqubits, ptr(ops), len(ops), F, C.byref(a), C.byref(b),
+                                  C.byref(c), ptr(pi), ptr(oi), ptr(m), a.value, b.value,
+                                  c.value))
+    return pi, oi, m[:c.value]

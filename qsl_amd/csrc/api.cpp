@@ -9,6 +9,7 @@
 #include <cstring>
 #include <random>
 
+#include "dense.h"
 #include "gate_math.h"
 
 using namespace qslhip;
@@ -549,8 +550,15 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
     return guarded([&] {
         require_handle(h);
         QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
-        QH_REQUIRE(mode == QSLHIP_MODE_PER_GATE || mode == QSLHIP_MODE_FUSED,
+        QH_REQUIRE(mode == QSLHIP_MODE_PER_GATE || mode == QSLHIP_MODE_FUSED ||
+                       mode == QSLHIP_MODE_DENSE,
                    QSLHIP_EINVAL, "unknown mode");
+        if (mode == QSLHIP_MODE_DENSE) {
+            QH_REQUIRE(!h->np && h->prec == 4, QSLHIP_EINVAL,
+                       "dense MFMA mode needs a full-vector FP32 handle");
+            QH_REQUIRE(h->n >= (unsigned)kDenseTile, QSLHIP_EINVAL,
+                       "dense MFMA mode needs at least 12 qubits");
+        }
         std::vector<GateDesc> gates;
         gates.reserve(nops);
         for (uint64_t i = 0; i < nops; i++) {
@@ -558,6 +566,20 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
             validate_gate(h, o.kind, o.q0, o.q1);
             gates.push_back(make_desc(h->prec, o.kind, o.q0,
                                       is_gate2(o.kind) ? o.q1 : 0, o.angle));
+        }
+        if (mode == QSLHIP_MODE_DENSE) {
+            // block matrices from FP64 coefficients, rounded once to FP32
+            std::vector<GateDesc> g64;
+            g64.reserve(nops);
+            for (uint64_t i = 0; i < nops; i++)
+                g64.push_back(make_desc(8, ops[i].kind, ops[i].q0,
+                                        is_gate2(ops[i].kind) ? ops[i].q1 : 0, ops[i].angle));
+            std::vector<DensePass> dp;
+            std::vector<DenseOp> dops;
+            std::vector<float> mats;
+            plan_dense_passes(h->n, g64, tuning().dense_f, dp, dops, mats);
+            launch_dense_passes(h, tuning().dense_f, dp, dops, mats);
+            return;
         }
         if (mode == QSLHIP_MODE_PER_GATE || h->np) {
             for (const GateDesc &g : gates) {
@@ -606,6 +628,65 @@ int qslhip_plan(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
             if (tile_masks) tile_masks[i] = passes[i].tile_mask;
             if (op_counts) op_counts[i] = passes[i].op_count;
         }
+    });
+}
+
+int qslhip_plan_dense(unsigned nqubits, const qslhip_op *ops, uint64_t nops, int F,
+                      uint64_t *npasses, uint64_t *nops_out, uint64_t *nfloats,
+                      uint64_t *pass_info, int64_t *op_info, float *mats,
+                      uint64_t cap_passes, uint64_t cap_ops, uint64_t cap_floats)
+{
+    return guarded([&] {
+        QH_REQUIRE(npasses && nops_out && nfloats, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
+        QH_REQUIRE(F == 4 || F == 5, QSLHIP_EINVAL, "fusion width must be 4 or 5");
+        QH_REQUIRE(nqubits >= (unsigned)kDenseTile && nqubits <= kMaxQubits, QSLHIP_EINVAL,
+                   "dense plans need 12..40 qubits");
+        std::vector<GateDesc> g64;
+        for (uint64_t i = 0; i < nops; i++) {
+            const qslhip_op &o = ops[i];
+            bool ok1 = is_gate1(o.kind) && o.q0 < nqubits;
+            bool ok2 = is_gate2(o.kind) && o.q0 < nqubits && o.q1 < nqubits && o.q0 != o.q1;
+            QH_REQUIRE(ok1 || ok2, QSLHIP_EINVAL, "bad op " + std::to_string(i));
+            g64.push_back(make_desc(8, o.kind, o.q0, is_gate2(o.kind) ? o.q1 : 0, o.angle));
+        }
+        std::vector<DensePass> dp;
+        std::vector<DenseOp> dops;
+        std::vector<float> m;
+        plan_dense_passes(nqubits, g64, F, dp, dops, m);
+        *npasses = dp.size();
+        *nops_out = dops.size();
+        *nfloats = m.size();
+        for (uint64_t i = 0; pass_info && i < dp.size() && i < cap_passes; i++) {
+            pass_info[3 * i] = dp[i].tile_mask;
+            pass_info[3 * i + 1] = dp[i].op_first;
+            pass_info[3 * i + 2] = dp[i].op_count;
+        }
+        for (uint64_t i = 0; op_info && i < dops.size() && i < cap_ops; i++) {
+            const DenseOp &o = dops[i];
+            int64_t *r = op_info + 9 * i;
+            r[0] = o.kind;
+            r[1] = o.gidx;
+            r[2] = o.kind ? o.nq : 0;
+            r[3] = o.kind ? (int64_t)o.mat * 2 : -1;
+            // tile positions -> qubits through the pass's tile mask
+            uint64_t mask = 0;
+            for (const DensePass &P : dp)
+                if (i >= P.op_first && i < P.op_first + P.op_count) mask = P.tile_mask;
+            for (int j = 0; j < 5; j++) {
+                int64_t q = -1;
+                if (o.kind && j < o.nq) {
+                    int pos = o.pos[j], seen = 0;
+                    for (unsigned b = 0; b < 64; b++)
+                        if ((mask >> b) & 1ull) {
+                            if (seen == pos) { q = b; break; }
+                            seen++;
+                        }
+                }
+                r[4 + j] = q;
+            }
+        }
+        if (mats && cap_floats) std::memcpy(mats, m.data(), std::min<uint64_t>(cap_floats, m.size()) * sizeof(float));
     });
 }
 

@@ -161,6 +161,7 @@ struct Tuning {
     int logt_f32 = 11, low_f32 = 5;
     int lds_max_target_f64 = 3;  // per-gate path: targets below use a tile
     int lds_max_target_f32 = 4;
+    int dense_f = 5;             // qubits per fused dense unitary (4 or 5)
 };
 Tuning &tuning();
 

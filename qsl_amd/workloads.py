@@ -142,12 +142,29 @@ def c4_observable(n):
 
 
 def c4_batch(ncirc=4096, layers=8, stream=None):
-    """Config 4: ncirc circuits with n_c = 10 + floor(11 u) qubits."""
+    """Config 4: ncirc circuits with n_c = 10 + floor(11 u) qubits, each the
+    ansatz of c4_ansatz drawn from the same stream in sequence (one u for
+    n_c, then 2 n_c angles per layer).  Vectorised: same ops as calling
+    c4_ansatz circuit by circuit."""
     s = stream or Stream(4)
     circuits = []
     for _ in range(ncirc):
         n = 10 + min(10, int(11 * s.take()))
-        ops, _ = c4_ansatz(n, layers, s)
+        ang = np.pi * (2 * s.take(2 * n * layers) - 1)
+        per = 3 * n - 1
+        ops = np.zeros(layers * per, OP_DTYPE)
+        q = np.arange(n)
+        for l in range(layers):
+            base = l * per
+            ops["kind"][base:base + n] = L.ROT_Y
+            ops["q0"][base:base + n] = q
+            ops["angle"][base:base + n] = ang[2 * n * l:2 * n * l + n]
+            ops["kind"][base + n:base + 2 * n] = L.ROT_Z
+            ops["q0"][base + n:base + 2 * n] = q
+            ops["angle"][base + n:base + 2 * n] = ang[2 * n * l + n:2 * n * (l + 1)]
+            ops["kind"][base + 2 * n:base + per] = L.CNOT
+            ops["q0"][base + 2 * n:base + per] = q[:-1]
+            ops["q1"][base + 2 * n:base + per] = q[1:]
         circuits.append((n, ops, c4_observable(n)))
     return circuits
 
