@@ -76,6 +76,17 @@ float time_apply(qslhip_t h, const std::vector<qslhip_op> &ops, int mode, Timer 
     return ms;
 }
 
+// A tile width the fused path can run: the LDS kernel keeps the tile and
+// an offset table within 64 KiB; register tiles (10..12 tile qubits) use
+// LDS only as the 2^logt-amplitude remap buffer.
+bool tile_fits(const Tuning &t, int logt, int low, int ab)
+{
+    if (logt < 6 || logt > 13 || low < 1 || low > logt - 2 || logt - low > 8) return false;
+    const size_t tile = (size_t)ab << logt;
+    if (t.reg_tile && logt >= 10 && logt <= 12) return tile <= 64 * 1024;
+    return tile <= 64 * 1024 - 2048;
+}
+
 }  // namespace
 
 extern "C" {
@@ -93,6 +104,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "lds_max_target_f64") *value = t.lds_max_target_f64;
         else if (k == "lds_max_target_f32") *value = t.lds_max_target_f32;
         else if (k == "dense_f") *value = t.dense_f;
+        else if (k == "reg_tile") *value = t.reg_tile;
         else throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
     });
 }
@@ -103,10 +115,7 @@ int qslhip_tuning_set(const char *key, int value)
         QH_REQUIRE(key, QSLHIP_EINVAL, "null key");
         Tuning &t = tuning();
         std::string k = key;
-        auto tile_ok = [](int logt, int low, int ab) {
-            return logt >= 6 && logt <= 13 && low >= 1 && low <= logt - 2 && logt - low <= 8 &&
-                   ((size_t)ab << logt) <= 64 * 1024 - 2048;
-        };
+        auto tile_ok = [&](int logt, int low, int ab) { return tile_fits(t, logt, low, ab); };
         if (k == "logt_f64") {
             QH_REQUIRE(tile_ok(value, t.low_f64, 16), QSLHIP_EINVAL, "bad logt_f64");
             t.logt_f64 = value;
@@ -128,6 +137,14 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "dense_f") {
             QH_REQUIRE(value == 4 || value == 5, QSLHIP_EINVAL, "dense_f must be 4 or 5");
             t.dense_f = value;
+        } else if (k == "reg_tile") {
+            QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "reg_tile must be 0 or 1");
+            Tuning u = t;
+            u.reg_tile = value;
+            QH_REQUIRE(tile_fits(u, u.logt_f64, u.low_f64, 16) &&
+                           tile_fits(u, u.logt_f32, u.low_f32, 8),
+                       QSLHIP_EINVAL, "current tile widths need register tiles");
+            t.reg_tile = value;
         } else {
             throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
         }
@@ -156,9 +173,13 @@ int qslhip_autotune(int prec, unsigned nqubits, char *report, size_t cap)
             float best = 1e30f;
             const int ab = prec == 8 ? 16 : 8;
             bool first = true;
-            for (int logt = 9; logt <= 13; logt++) {
-                for (int low = 3; low <= 7; low++) {
-                    if (logt - low > 8 || ((size_t)ab << logt) > 64 * 1024 - 2048) continue;
+            int best_r = t.reg_tile;
+            for (int cand = 0; cand < 2 * 5 * 5; cand++) {
+                const int reg = 1 - cand / 25, logt = 9 + (cand / 5) % 5, low = 2 + cand % 5;
+                {
+                    t.reg_tile = reg;
+                    if (!tile_fits(t, logt, low, ab) || (reg && (logt < 10 || logt > 12)))
+                        continue;
                     if (prec == 8) {
                         t.logt_f64 = logt;
                         t.low_f64 = low;
@@ -168,17 +189,19 @@ int qslhip_autotune(int prec, unsigned nqubits, char *report, size_t cap)
                     }
                     float ms = time_apply(h, ops, QSLHIP_MODE_FUSED, tm);
                     char buf[96];
-                    std::snprintf(buf, sizeof buf, "%s[%d, %d, %.4f]", first ? "" : ", ", logt,
-                                  low, ms);
+                    std::snprintf(buf, sizeof buf, "%s[%d, %d, %d, %.4f]", first ? "" : ", ",
+                                  reg, logt, low, ms);
                     rep += buf;
                     first = false;
                     if (ms < best) {
                         best = ms;
                         best_l = logt;
                         best_c = low;
+                        best_r = reg;
                     }
                 }
             }
+            t.reg_tile = best_r;
             if (prec == 8) {
                 t.logt_f64 = best_l;
                 t.low_f64 = best_c;
@@ -186,8 +209,17 @@ int qslhip_autotune(int prec, unsigned nqubits, char *report, size_t cap)
                 t.logt_f32 = best_l;
                 t.low_f32 = best_c;
             }
-            rep += "], \"tile_best\": [" + std::to_string(best_l) + ", " +
-                   std::to_string(best_c) + "]";
+            // the other precision's widths must stay runnable
+            if (!tile_fits(t, t.logt_f64, t.low_f64, 16)) {
+                t.logt_f64 = 10;
+                t.low_f64 = 3;
+            }
+            if (!tile_fits(t, t.logt_f32, t.low_f32, 8)) {
+                t.logt_f32 = 11;
+                t.low_f32 = 3;
+            }
+            rep += "], \"tile_best\": [" + std::to_string(best_r) + ", " +
+                   std::to_string(best_l) + ", " + std::to_string(best_c) + "]";
             // per-gate threshold: one-op tile vs streaming, target by target
             int thr = -1;
             rep += ", \"per_gate\": [";

@@ -18,6 +18,7 @@ struct DenseOp {
     int32_t pos[5];
     uint32_t mat;
     uint32_t gidx;  // first gate of the op in the input list (introspection)
+    uint32_t frag;  // float offset of its MFMA A-operand fragments (device)
     TileOp top;
 };
 

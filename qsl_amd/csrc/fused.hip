@@ -128,6 +128,12 @@ void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
         else run<float>(h, passes, nullptr, amps, &ops[0]);
         return;
     }
+    bool reg = tuning().reg_tile != 0;
+    for (const TilePass &p : passes) reg = reg && reg_tile_ok(p);
+    if (reg) {
+        launch_reg_passes(h, passes, ops, amps);
+        return;
+    }
     size_t bytes = ops.size() * sizeof(TileOp);
     // The pinned staging buffer may still be read by the previous copy.
     if (h->staged) QH_CHECK(hipEventSynchronize(h->staged));

@@ -119,6 +119,11 @@ namespace qslhip {
 void launch_gate(qslhip_state *h, const GateDesc &g, void *amps = nullptr);
 void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                         const std::vector<TileOp> &ops, void *amps = nullptr);
+// the same passes with the amplitudes held in registers (regtile.hip);
+// every pass must have 10..12 tile qubits
+void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
+                       const std::vector<TileOp> &ops, void *amps);
+bool reg_tile_ok(const TilePass &p);
 void launch_fill_basis(qslhip_state *h, uint64_t index);
 double device_prob(qslhip_state *h, unsigned t, unsigned outcome);
 double device_norm2(qslhip_state *h);
@@ -157,11 +162,12 @@ bool is_np_kind(int kind);
 
 // Tunables picked by the autotuner (autotune.cpp), with safe defaults.
 struct Tuning {
-    int logt_f64 = 10, low_f64 = 4;
-    int logt_f32 = 11, low_f32 = 5;
+    int logt_f64 = 10, low_f64 = 3;
+    int logt_f32 = 11, low_f32 = 3;
     int lds_max_target_f64 = 3;  // per-gate path: targets below use a tile
     int lds_max_target_f32 = 4;
     int dense_f = 5;             // qubits per fused dense unitary (4 or 5)
+    int reg_tile = 0;            // fused passes hold amplitudes in registers
 };
 Tuning &tuning();
 

@@ -15,6 +15,13 @@
 // Per amplitude a dense op costs 8 * 2^F real flops (256 for F = 5) against
 // 16 bytes of HBM traffic for the whole pass, so a pass holding two or more
 // 5-qubit unitaries is matrix-core bound.
+//
+// The A operand of every (m-tile, k-step) is laid out on the host as the 64
+// floats the 64 lanes need, so a wave fetches it with one coalesced 256-byte
+// load (L2-resident: 16 KiB per 5-qubit unitary).  The LDS tile is stored
+// XOR-swizzled, element e at e ^ ((e >> 5) & 31): column vectors whose
+// stride is a multiple of 32 elements (targets on the low tile qubits) then
+// fall into distinct banks instead of one.
 #include <cstring>
 
 #include "dense.h"
@@ -38,10 +45,18 @@ __device__ __forceinline__ uint32_t deposit32(uint32_t v, uint32_t mask)
     return r;
 }
 
+struct Swizzle32 {
+    __device__ __forceinline__ uint32_t operator()(uint32_t e) const
+    {
+        return e ^ ((e >> 5) & 31u);
+    }
+};
+
 template <int F>
 __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const DenseOp &op,
-                                            const float2 *__restrict__ mats)
+                                            const float *__restrict__ frags)
 {
+    const Swizzle32 sw;
     constexpr int V = 1 << F;         // amplitudes per column vector
     constexpr int MT = (2 * V) / 32;  // 32-row tiles of the real embedding
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -55,7 +70,7 @@ __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const Dens
         tmask |= epos[j];
     }
     const uint32_t cmask = ((1u << K) - 1) & ~tmask;
-    const float2 *U = mats + op.mat;
+    const float *A = frags + op.frag + lane;
     float *tf = reinterpret_cast<float *>(tile);
     const int hi = lane >> 5, lo = lane & 31;
     for (int cb = wave; cb < nblk; cb += nwaves) {
@@ -65,18 +80,16 @@ __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const Dens
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
             for (int j = 0; j < 16; j++) acc[mt][j] = 0.f;
+#pragma unroll 4
         for (int ks = 0; ks < V; ks++) {
             uint32_t eo = 0;
 #pragma unroll
             for (int j = 0; j < F; j++)
                 if ((ks >> j) & 1) eo |= epos[j];
-            const float b = tf[2 * (colbase | eo) + hi];
+            const float b = tf[2 * sw(colbase | eo) + hi];
 #pragma unroll
             for (int mt = 0; mt < MT; mt++) {
-                const int row = mt * 32 + lo;
-                const int r = row >> 1, rp = row & 1;
-                const float2 u = U[r * V + ks];
-                const float a = (rp == hi) ? u.x : (rp == 0 ? -u.y : u.y);
+                const float a = A[(mt * V + ks) * 64];
                 acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[mt], 0, 0, 0);
             }
         }
@@ -90,7 +103,7 @@ __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const Dens
 #pragma unroll
                 for (int q = 0; q < F; q++)
                     if ((r >> q) & 1) eo |= epos[q];
-                tf[2 * (colbase | eo) + comp] = acc[mt][j];
+                tf[2 * sw(colbase | eo) + comp] = acc[mt][j];
             }
         }
     }
@@ -100,8 +113,9 @@ template <int F>
 __global__ __launch_bounds__(kBlock) void k_dense(cplx<float> *__restrict__ st, unsigned n,
                                                   int K, int crun, uint64_t tile_mask,
                                                   const DenseOp *__restrict__ ops, int nops,
-                                                  const float2 *__restrict__ mats)
+                                                  const float *__restrict__ frags)
 {
+    const Swizzle32 sw;
     extern __shared__ __align__(16) unsigned char smem[];
     cplx<float> *tile = reinterpret_cast<cplx<float> *>(smem);
     uint64_t *hoff = reinterpret_cast<uint64_t *>(smem + (sizeof(cplx<float>) << K));
@@ -114,16 +128,16 @@ __global__ __launch_bounds__(kBlock) void k_dense(cplx<float> *__restrict__ st, 
     const uint32_t nt = 1u << K;
     const uint32_t lowm = (1u << crun) - 1;
     for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        tile[e] = st[gbase | (e & lowm) | hoff[e >> crun]];
+        tile[sw(e)] = st[gbase | (e & lowm) | hoff[e >> crun]];
     __syncthreads();
     for (int i = 0; i < nops; i++) {
         const DenseOp op = ops[i];
-        if (op.kind == 0) tile_op<float>(tile, K, gbase, op.top);
-        else dense_apply<F>(tile, K, op, mats);
+        if (op.kind == 0) tile_op<float>(tile, K, gbase, op.top, sw);
+        else dense_apply<F>(tile, K, op, frags);
         __syncthreads();
     }
     for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        st[gbase | (e & lowm) | hoff[e >> crun]] = tile[e];
+        st[gbase | (e & lowm) | hoff[e >> crun]] = tile[sw(e)];
 }
 
 }  // namespace
@@ -136,19 +150,40 @@ void launch_dense_passes(qslhip_state *h, int F, const std::vector<DensePass> &p
     if (passes.empty()) return;
     const int K = kDenseTile;
     QH_REQUIRE((int)h->n >= K, QSLHIP_EINVAL, "dense passes need at least 12 qubits");
-    size_t ob = ops.size() * sizeof(DenseOp);
+    // A-operand fragments: for dense op d, m-tile mt, k-step ks, lane l the
+    // float A[mt*32 + l%32][2*ks + l/32] of the interleaved real embedding
+    // of U (rows 2r / 2r+1 = Re / Im of y_r, columns 2s / 2s+1 = Re / Im of
+    // x_s): (0,0) Re U, (0,1) -Im U, (1,0) Im U, (1,1) Re U.
+    std::vector<DenseOp> hops(ops);
+    std::vector<float> frags;
+    const int V = 1 << F, MT = 2 * V / 32;
+    for (DenseOp &o : hops) {
+        if (o.kind != 1) continue;
+        o.frag = (uint32_t)frags.size();
+        const float *U = mats.data() + 2 * (size_t)o.mat;
+        for (int mt = 0; mt < MT; mt++)
+            for (int ks = 0; ks < V; ks++)
+                for (int l = 0; l < 64; l++) {
+                    const int row = mt * 32 + (l & 31), kp = l >> 5;
+                    const int r = row >> 1, rp = row & 1;
+                    const float ur = U[2 * ((size_t)r * V + ks)];
+                    const float ui = U[2 * ((size_t)r * V + ks) + 1];
+                    frags.push_back(rp == kp ? ur : (rp == 0 ? -ui : ui));
+                }
+    }
+    size_t ob = hops.size() * sizeof(DenseOp);
     size_t ob_al = (ob + 255) & ~(size_t)255;
-    size_t mb = mats.size() * sizeof(float);
+    size_t mb = frags.size() * sizeof(float);
     if (h->staged) QH_CHECK(hipEventSynchronize(h->staged));
     h->pin.ensure(ob_al + mb + 16);
     h->optab.ensure(ob_al + mb + 16);
-    std::memcpy(h->pin.ptr, ops.data(), ob);
-    if (mb) std::memcpy(static_cast<char *>(h->pin.ptr) + ob_al, mats.data(), mb);
+    std::memcpy(h->pin.ptr, hops.data(), ob);
+    if (mb) std::memcpy(static_cast<char *>(h->pin.ptr) + ob_al, frags.data(), mb);
     QH_CHECK(hipMemcpyAsync(h->optab.ptr, h->pin.ptr, ob_al + mb, hipMemcpyHostToDevice,
                             h->stream));
     QH_CHECK(hipEventRecord(h->staged, h->stream));
     const DenseOp *dops = static_cast<const DenseOp *>(h->optab.ptr);
-    const float2 *dm = reinterpret_cast<const float2 *>(static_cast<char *>(h->optab.ptr) + ob_al);
+    const float *dm = reinterpret_cast<const float *>(static_cast<char *>(h->optab.ptr) + ob_al);
     for (const DensePass &p : passes) {
         int crun = 0;
         while (crun < 64 && ((p.tile_mask >> crun) & 1ull)) crun++;

@@ -28,9 +28,14 @@ __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
     return r;
 }
 
-template <typename T>
+// Identity element order of an LDS tile (see mfma.hip for a swizzled one).
+struct NoSwizzle {
+    __device__ __forceinline__ uint32_t operator()(uint32_t e) const { return e; }
+};
+
+template <typename T, typename SW = NoSwizzle>
 __device__ __forceinline__ void tile_op(cplx<T> *tile, int K, uint64_t gbase,
-                                        const TileOp &op)
+                                        const TileOp &op, SW sw = SW())
 {
     const int tid = threadIdx.x;
     const uint32_t kBlock = blockDim.x;
@@ -47,24 +52,24 @@ __device__ __forceinline__ void tile_op(cplx<T> *tile, int K, uint64_t gbase,
         if (pa >= 0) {
             for (uint32_t f = tid; f < nt / 2; f += kBlock) {
                 uint32_t j0 = (uint32_t)ins0(f, pa), j1 = j0 | (1u << pa);
-                cplx<T> a = tile[j0], b = tile[j1];
+                cplx<T> a = tile[sw(j0)], b = tile[sw(j1)];
                 pair_update<T>(body, a, b, c, s);
-                tile[j0] = a;
-                tile[j1] = b;
+                tile[sw(j0)] = a;
+                tile[sw(j1)] = b;
             }
         } else {  // rotateZ on a non-tile qubit: one half-formula for all
             bool one = gbit(op.q0);
-            for (uint32_t j = tid; j < nt; j += kBlock) rz_half<T>(one, tile[j], c, s);
+            for (uint32_t j = tid; j < nt; j += kBlock) rz_half<T>(one, tile[sw(j)], c, s);
         }
         break;
     case PC_ONE:
         if (pa >= 0) {
             for (uint32_t f = tid; f < nt / 2; f += kBlock) {
                 uint32_t j = (uint32_t)ins0(f, pa) | (1u << pa);
-                one_update<T>(body, tile[j], c, s);
+                one_update<T>(body, tile[sw(j)], c, s);
             }
         } else if (gbit(op.q0)) {
-            for (uint32_t j = tid; j < nt; j += kBlock) one_update<T>(body, tile[j], c, s);
+            for (uint32_t j = tid; j < nt; j += kBlock) one_update<T>(body, tile[sw(j)], c, s);
         }
         break;
     case PC_CTRL:
@@ -75,18 +80,18 @@ __device__ __forceinline__ void tile_op(cplx<T> *tile, int K, uint64_t gbase,
                 for (uint32_t f = tid; f < nt / 4; f += kBlock) {
                     uint32_t base = (uint32_t)ins0(ins0(f, lo), hi);
                     uint32_t j0 = base | (1u << pa), j1 = j0 | (1u << pb);
-                    cplx<T> a = tile[j0], b = tile[j1];
+                    cplx<T> a = tile[sw(j0)], b = tile[sw(j1)];
                     pair_update<T>(body, a, b, c, s);
-                    tile[j0] = a;
-                    tile[j1] = b;
+                    tile[sw(j0)] = a;
+                    tile[sw(j1)] = b;
                 }
             } else {
                 for (uint32_t f = tid; f < nt / 2; f += kBlock) {
                     uint32_t j0 = (uint32_t)ins0(f, pb), j1 = j0 | (1u << pb);
-                    cplx<T> a = tile[j0], b = tile[j1];
+                    cplx<T> a = tile[sw(j0)], b = tile[sw(j1)];
                     pair_update<T>(body, a, b, c, s);
-                    tile[j0] = a;
-                    tile[j1] = b;
+                    tile[sw(j0)] = a;
+                    tile[sw(j1)] = b;
                 }
             }
         } else {  // controlRotateZ with its target outside the tile
@@ -94,10 +99,10 @@ __device__ __forceinline__ void tile_op(cplx<T> *tile, int K, uint64_t gbase,
             if (pa >= 0) {
                 for (uint32_t f = tid; f < nt / 2; f += kBlock) {
                     uint32_t j = (uint32_t)ins0(f, pa) | (1u << pa);
-                    rz_half<T>(one, tile[j], c, s);
+                    rz_half<T>(one, tile[sw(j)], c, s);
                 }
             } else {
-                for (uint32_t j = tid; j < nt; j += kBlock) rz_half<T>(one, tile[j], c, s);
+                for (uint32_t j = tid; j < nt; j += kBlock) rz_half<T>(one, tile[sw(j)], c, s);
             }
         }
         break;
@@ -108,16 +113,16 @@ __device__ __forceinline__ void tile_op(cplx<T> *tile, int K, uint64_t gbase,
             int lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
             for (uint32_t f = tid; f < nt / 4; f += kBlock) {
                 uint32_t j = (uint32_t)ins0(ins0(f, lo), hi) | (1u << pa) | (1u << pb);
-                one_update<T>(body, tile[j], c, s);
+                one_update<T>(body, tile[sw(j)], c, s);
             }
         } else if (pa >= 0 || pb >= 0) {
             int p = pa >= 0 ? pa : pb;
             for (uint32_t f = tid; f < nt / 2; f += kBlock) {
                 uint32_t j = (uint32_t)ins0(f, p) | (1u << p);
-                one_update<T>(body, tile[j], c, s);
+                one_update<T>(body, tile[sw(j)], c, s);
             }
         } else {
-            for (uint32_t j = tid; j < nt; j += kBlock) one_update<T>(body, tile[j], c, s);
+            for (uint32_t j = tid; j < nt; j += kBlock) one_update<T>(body, tile[sw(j)], c, s);
         }
         break;
     }
@@ -127,13 +132,13 @@ __device__ __forceinline__ void tile_op(cplx<T> *tile, int K, uint64_t gbase,
         for (uint32_t f = tid; f < nt / 4; f += kBlock) {
             uint32_t base = (uint32_t)ins0(ins0(f, lo), hi);
             uint32_t j0 = base | (1u << pa), j1 = base | (1u << pb);
-            cplx<T> a = tile[j0], b = tile[j1];
+            cplx<T> a = tile[sw(j0)], b = tile[sw(j1)];
             pair_update<T>(body, a, b, c, s);
-            tile[j0] = a;
-            tile[j1] = b;
+            tile[sw(j0)] = a;
+            tile[sw(j1)] = b;
             if (cls == PC_FSWAP) {
                 uint32_t j = base | (1u << pa) | (1u << pb);
-                one_update<T>(QSLHIP_PAULI_Z, tile[j], c, s);
+                one_update<T>(QSLHIP_PAULI_Z, tile[sw(j)], c, s);
             }
         }
         break;

@@ -76,11 +76,13 @@ def test_dense_mode_validation():
 
 
 def test_autotune_sets_valid_widths():
-    saved = {k: L.tuning_get(k) for k in ("logt_f32", "low_f32", "lds_max_target_f32",
-                                          "dense_f")}
+    keys = ("logt_f64", "low_f64", "logt_f32", "low_f32", "lds_max_target_f32", "dense_f")
+    saved = {k: L.tuning_get(k) for k in keys}
+    saved_reg = L.tuning_get("reg_tile")
     try:
         rep = L.autotune(4, 22)
-        assert rep["tile_best"][0] == L.tuning_get("logt_f32")
+        assert rep["tile_best"] == [L.tuning_get("reg_tile"), L.tuning_get("logt_f32"),
+                                    L.tuning_get("low_f32")]
         assert L.tuning_get("dense_f") in (4, 5)
         # the tuned configuration still computes the same circuit
         n = 16
@@ -91,5 +93,7 @@ def test_autotune_sets_valid_widths():
         b.apply_ops(ops, mode=qsl_amd.MODE_PER_GATE)
         assert np.array_equal(a.getState().view(np.uint8), b.getState().view(np.uint8))
     finally:
+        L.tuning_set("reg_tile", 1)  # admits every width the LDS kernel admits
         for k, v in saved.items():
             L.tuning_set(k, v)
+        L.tuning_set("reg_tile", saved_reg)

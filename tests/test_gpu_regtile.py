@@ -1,0 +1,119 @@
+"""Register-resident tile passes (the default fused path for 10..12 tile
+qubits) against the oracle, bit for bit.
+
+Each pass keeps 2^(K-8) amplitudes per thread in registers and goes through
+LDS only to remap which tile qubits index the registers, as planned on the
+host for the targets that follow.  Every tile width, contiguous low run and
+both precisions run the same random all-kinds circuit; the LDS-resident
+kernel (reg_tile = 0) runs it too, as a second implementation that must
+agree bit for bit.
+"""
+import numpy as np
+import pytest
+
+import qsl_amd
+from qsl_amd import _lib as L
+from oracle.pyoracle import Oracle, OP_DTYPE, GATE1, GATE2, random_state
+
+pytestmark = pytest.mark.gpu
+
+O = Oracle()
+
+
+def _set_widths(sfx, logt, low):
+    # tile widths are validated pairwise; one of the two orders is legal
+    try:
+        L.tuning_set("logt_" + sfx, logt)
+        L.tuning_set("low_" + sfx, low)
+    except qsl_amd.QslHipError:
+        L.tuning_set("low_" + sfx, low)
+        L.tuning_set("logt_" + sfx, logt)
+
+
+@pytest.fixture
+def tuning():
+    saved = {s: (L.tuning_get("logt_" + s), L.tuning_get("low_" + s)) for s in ("f64", "f32")}
+    saved_reg = L.tuning_get("reg_tile")
+
+    def setter(reg, prec, logt, low):
+        L.tuning_set("reg_tile", 1)  # admits every width the LDS kernel admits
+        _set_widths("f64" if prec == 8 else "f32", logt, low)
+        L.tuning_set("reg_tile", reg)
+
+    yield setter
+    L.tuning_set("reg_tile", 1)
+    for s, (logt, low) in saved.items():
+        _set_widths(s, logt, low)
+    L.tuning_set("reg_tile", saved_reg)
+
+
+def _ops(rng, n, count, dtype):
+    kinds = GATE1 + GATE2
+    ops = np.zeros(count, OP_DTYPE)
+    for i in range(count):
+        k = int(rng.choice(kinds))
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        ang = float(rng.uniform(-2 * np.pi, 2 * np.pi))
+        if dtype == np.complex64:
+            ang = float(np.float32(ang))
+        ops[i] = (k, a, b if k >= 16 else 0, 0, ang)
+    return ops
+
+
+CONFIGS = [(1, 10, 2), (1, 10, 3), (1, 11, 3), (1, 11, 5), (1, 12, 4), (1, 12, 6),
+           (0, 10, 3), (0, 11, 3)]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [12, 15])
+@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "reg%d_t%d_low%d" % c)
+def test_tile_kernels_match_oracle(tuning, dtype, n, cfg):
+    reg, logt, low = cfg
+    prec = 8 if dtype == np.complex128 else 4
+    tuning(reg, prec, logt, low)
+    rng = np.random.default_rng(1000 * logt + 10 * low + n + prec)
+    st0 = random_state(rng, n, dtype)
+    ops = _ops(rng, n, 300, dtype)
+    q = qsl_amd.Qubits(n, dtype=dtype)
+    q.setState(st0)
+    q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    got = q.getState()
+    want = st0.copy()
+    O.apply_ops(want, ops)
+    assert np.abs(got - want).max() <= (1e-12 if prec == 8 else 1e-5)
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), cfg
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_register_remaps_on_every_qubit(tuning, dtype):
+    # targets sweep every tile position in both directions, so each layout
+    # is left for every other one; controls and diagonals on thread bits,
+    # register bits and group bits
+    n = 18
+    prec = 8 if dtype == np.complex128 else 4
+    tuning(1, prec, 12, 4)
+    rng = np.random.default_rng(18)
+    seq = []
+    for t in list(range(n)) + list(range(n - 1, -1, -1)):
+        seq.append((5, t, 0, 0, 0.37 + t))
+        seq.append((16, (t + 5) % n, t, 0, 0.0))
+        seq.append((21, t, (t + 9) % n, 0, 1.1 * t))
+        seq.append((22, (t + 3) % n, t, 0, 0.5))
+        seq.append((24, t, (t + 1) % n, 0, 0.0))
+        seq.append((26, (t + 2) % n, t, 0, 0.9))
+    ops = np.array(seq, OP_DTYPE)
+    if dtype == np.complex64:
+        ops["angle"] = ops["angle"].astype(np.float32)
+    st0 = random_state(rng, n, dtype)
+    q = qsl_amd.Qubits(n, dtype=dtype)
+    q.setState(st0)
+    q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    want = st0.copy()
+    O.apply_ops(want, ops)
+    assert np.array_equal(q.getState().view(np.uint8), want.view(np.uint8))
+
+
+def test_tuning_rejects_lds_widths_without_register_tiles(tuning):
+    tuning(1, 8, 12, 4)  # 2^12 FP64 amplitudes: only the register kernel fits
+    with pytest.raises(qsl_amd.QslHipError):
+        L.tuning_set("reg_tile", 0)
