@@ -66,9 +66,15 @@ enum {
     QSLHIP_MODE_PER_GATE = 0, /* one kernel per gate (the reference's loop)  */
     QSLHIP_MODE_FUSED = 1,    /* gates packed into LDS-tile passes; same
                                  per-amplitude arithmetic, bit-identical     */
-    QSLHIP_MODE_DENSE = 2     /* FP32 handles: gate runs fused into dense
+    QSLHIP_MODE_DENSE = 2,    /* FP32 handles: gate runs fused into dense
                                  4/5-qubit unitaries (FP64 products rounded
                                  once) applied on matrix cores (MFMA)        */
+    QSLHIP_MODE_REORDER = 3   /* tile passes after commutation-aware
+                                 reordering (gates on disjoint qubits, or
+                                 both diagonal, may swap): the same product
+                                 in exact arithmetic, fewer HBM passes; the
+                                 amplitudes differ from program order by
+                                 floating-point rounding only               */
 };
 
 typedef struct qslhip_state *qslhip_t;
@@ -149,6 +155,12 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops,
 int qslhip_plan(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
                 uint64_t *npasses, uint64_t *tile_masks, uint64_t *op_counts,
                 uint64_t cap);
+/* The same for any tile mode (QSLHIP_MODE_FUSED or QSLHIP_MODE_REORDER),
+ * also writing the execution order: order[i] = index in `ops` of the i-th
+ * gate applied (nops entries; may be null). */
+int qslhip_plan_mode(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
+                     int mode, uint64_t *npasses, uint64_t *tile_masks,
+                     uint64_t *op_counts, uint64_t cap, uint64_t *order);
 
 /* ---- tuning (no reference counterpart) ------------------------------------
  * Keys: "logt_f64", "low_f64", "logt_f32", "low_f32" (LDS tile width and the

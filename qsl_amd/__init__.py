@@ -6,7 +6,8 @@ package is the Python mirror of the same API used by the tests and
 bench.py; it has no CPU fallback.
 """
 from ._lib import (QslHipError, SizeError, NotNPError, uniforms, device_count,  # noqa: F401
-                   OP_DTYPE, MODE_PER_GATE, MODE_FUSED, MODE_DENSE, LIB_PATH, HEADER_PATH)
+                   OP_DTYPE, MODE_PER_GATE, MODE_FUSED, MODE_DENSE, MODE_REORDER, LIB_PATH,
+                   HEADER_PATH)
 from . import _lib as abi  # noqa: F401
 from .qubits import Qubits, QubitsNP, Batch  # noqa: F401
 from ._lib import PAULI_DTYPE  # noqa: F401

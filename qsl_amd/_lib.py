@@ -21,7 +21,7 @@ PAULI_X, PAULI_Y, PAULI_Z, HADAMARD, PHASE, ROT_X, ROT_Y, ROT_Z = range(8)
 CNOT, CY, CZ, CROT_X, CROT_Y, CROT_Z, CPHASE, CHADAMARD = range(16, 24)
 SWAP, FSWAP, NP_ROT_X, NP_ROT_Y, NP_HADAMARD = range(24, 29)
 
-MODE_PER_GATE, MODE_FUSED, MODE_DENSE = 0, 1, 2
+MODE_PER_GATE, MODE_FUSED, MODE_DENSE, MODE_REORDER = 0, 1, 2, 3
 
 OP_DTYPE = np.dtype([("kind", "<i4"), ("q0", "<u4"), ("q1", "<u4"),
                      ("pad", "<u4"), ("angle", "<f8")])
@@ -64,6 +64,7 @@ SIGNATURES = {
     "qslhip_get_stream": (_I, [_vp, C.POINTER(_vp)]),
     "qslhip_sync": (_I, [_vp]),
     "qslhip_plan": (_I, [_U, _I, _vp, _u64, C.POINTER(_u64), _vp, _vp, _u64]),
+    "qslhip_plan_mode": (_I, [_U, _I, _vp, _u64, _I, C.POINTER(_u64), _vp, _vp, _u64, _vp]),
     "qslhip_reset": (_I, [_vp]),
     "qslhip_set_state": (_I, [_vp, _vp, _u64]),
     "qslhip_get_state": (_I, [_vp, _vp, _u64]),
@@ -155,6 +156,20 @@ def plan(nqubits, prec, ops, cap=4096):
                             ptr(masks), ptr(counts), cap))
     m = min(cap, npasses.value)
     return masks[:m].copy(), counts[:m].copy()
+
+
+def plan_order(nqubits, prec, ops, mode=MODE_REORDER, cap=4096):
+    """Tile passes and execution order of qslhip_apply_ops(mode):
+    (masks, op_counts, order) with order[i] the index of the i-th gate run."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    npasses = _u64(0)
+    masks = np.zeros(cap, np.uint64)
+    counts = np.zeros(cap, np.uint64)
+    order = np.zeros(max(1, len(ops)), np.uint64)
+    check(lib().qslhip_plan_mode(nqubits, prec, ptr(ops), len(ops), mode, C.byref(npasses),
+                                 ptr(masks), ptr(counts), cap, ptr(order)))
+    m = min(cap, npasses.value)
+    return masks[:m].copy(), counts[:m].copy(), order[:len(ops)].copy()
 
 
 def tuning_get(key):

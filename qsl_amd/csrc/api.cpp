@@ -551,7 +551,7 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
         require_handle(h);
         QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
         QH_REQUIRE(mode == QSLHIP_MODE_PER_GATE || mode == QSLHIP_MODE_FUSED ||
-                       mode == QSLHIP_MODE_DENSE,
+                       mode == QSLHIP_MODE_DENSE || mode == QSLHIP_MODE_REORDER,
                    QSLHIP_EINVAL, "unknown mode");
         if (mode == QSLHIP_MODE_DENSE) {
             QH_REQUIRE(!h->np && h->prec == 4, QSLHIP_EINVAL,
@@ -593,7 +593,12 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
         int low = h->prec == 8 ? tu.low_f64 : tu.low_f32;
         std::vector<TilePass> passes;
         std::vector<TileOp> tops;
-        plan_tile_passes(h->n, h->prec, gates, logt, low, passes, tops);
+        if (mode == QSLHIP_MODE_REORDER) {
+            std::vector<uint32_t> order;
+            plan_tile_passes_reorder(h->n, gates, logt, low, passes, tops, order);
+        } else {
+            plan_tile_passes(h->n, h->prec, gates, logt, low, passes, tops);
+        }
         launch_tile_passes(h, passes, tops);
     });
 }
@@ -602,7 +607,17 @@ int qslhip_plan(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
                 uint64_t *npasses, uint64_t *tile_masks, uint64_t *op_counts,
                 uint64_t cap)
 {
+    return qslhip_plan_mode(nqubits, prec, ops, nops, QSLHIP_MODE_FUSED, npasses, tile_masks,
+                            op_counts, cap, nullptr);
+}
+
+int qslhip_plan_mode(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
+                     int mode, uint64_t *npasses, uint64_t *tile_masks,
+                     uint64_t *op_counts, uint64_t cap, uint64_t *order)
+{
     return guarded([&] {
+        QH_REQUIRE(mode == QSLHIP_MODE_FUSED || mode == QSLHIP_MODE_REORDER, QSLHIP_EINVAL,
+                   "plan mode must be FUSED or REORDER");
         QH_REQUIRE(npasses, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
         QH_REQUIRE(prec == 4 || prec == 8, QSLHIP_EINVAL, "prec must be 4 or 8");
@@ -622,7 +637,15 @@ int qslhip_plan(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops,
         int low = prec == 8 ? tu.low_f64 : tu.low_f32;
         std::vector<TilePass> passes;
         std::vector<TileOp> tops;
-        plan_tile_passes(nqubits, prec, gates, logt, low, passes, tops);
+        std::vector<uint32_t> ord;
+        if (mode == QSLHIP_MODE_REORDER) {
+            plan_tile_passes_reorder(nqubits, gates, logt, low, passes, tops, ord);
+        } else {
+            plan_tile_passes(nqubits, prec, gates, logt, low, passes, tops);
+            for (uint64_t i = 0; i < nops; i++) ord.push_back((uint32_t)i);
+        }
+        if (order)
+            for (size_t i = 0; i < ord.size(); i++) order[i] = ord[i];
         *npasses = passes.size();
         for (uint64_t i = 0; i < passes.size() && i < cap; i++) {
             if (tile_masks) tile_masks[i] = passes[i].tile_mask;

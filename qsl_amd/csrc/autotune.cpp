@@ -82,9 +82,10 @@ float time_apply(qslhip_t h, const std::vector<qslhip_op> &ops, int mode, Timer 
 bool tile_fits(const Tuning &t, int logt, int low, int ab)
 {
     if (logt < 6 || logt > 13 || low < 1 || low > logt - 2 || logt - low > 8) return false;
-    const size_t tile = (size_t)ab << logt;
-    if (t.reg_tile && logt >= 10 && logt <= 12) return tile <= 64 * 1024;
-    return tile <= 64 * 1024 - 2048;
+    const int prec = ab == 16 ? 8 : 4;
+    if (t.reg_tile && reg_tile_ok(logt, prec == 8 ? t.rthreads_f64 : t.rthreads_f32, prec))
+        return true;
+    return ((size_t)ab << logt) <= 64 * 1024 - 2048;
 }
 
 }  // namespace
@@ -105,6 +106,9 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "lds_max_target_f32") *value = t.lds_max_target_f32;
         else if (k == "dense_f") *value = t.dense_f;
         else if (k == "reg_tile") *value = t.reg_tile;
+        else if (k == "rthreads_f64") *value = t.rthreads_f64;
+        else if (k == "rthreads_f32") *value = t.rthreads_f32;
+        else if (k == "rgroups") *value = t.rgroups;
         else throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
     });
 }
@@ -145,6 +149,17 @@ int qslhip_tuning_set(const char *key, int value)
                            tile_fits(u, u.logt_f32, u.low_f32, 8),
                        QSLHIP_EINVAL, "current tile widths need register tiles");
             t.reg_tile = value;
+        } else if (k == "rgroups") {
+            QH_REQUIRE(value >= 0 && value <= 1024, QSLHIP_EINVAL, "rgroups must be in [0, 1024]");
+            t.rgroups = value;
+        } else if (k == "rthreads_f64" || k == "rthreads_f32") {
+            QH_REQUIRE(value == 256 || value == 512, QSLHIP_EINVAL, "threads must be 256 or 512");
+            Tuning u = t;
+            (k == "rthreads_f64" ? u.rthreads_f64 : u.rthreads_f32) = value;
+            QH_REQUIRE(tile_fits(u, u.logt_f64, u.low_f64, 16) &&
+                           tile_fits(u, u.logt_f32, u.low_f32, 8),
+                       QSLHIP_EINVAL, "current tile widths need the other workgroup size");
+            t = u;
         } else {
             throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
         }
@@ -173,35 +188,41 @@ int qslhip_autotune(int prec, unsigned nqubits, char *report, size_t cap)
             float best = 1e30f;
             const int ab = prec == 8 ? 16 : 8;
             bool first = true;
-            int best_r = t.reg_tile;
-            for (int cand = 0; cand < 2 * 5 * 5; cand++) {
-                const int reg = 1 - cand / 25, logt = 9 + (cand / 5) % 5, low = 2 + cand % 5;
-                {
-                    t.reg_tile = reg;
-                    if (!tile_fits(t, logt, low, ab) || (reg && (logt < 10 || logt > 12)))
-                        continue;
-                    if (prec == 8) {
-                        t.logt_f64 = logt;
-                        t.low_f64 = low;
-                    } else {
-                        t.logt_f32 = logt;
-                        t.low_f32 = low;
-                    }
-                    float ms = time_apply(h, ops, QSLHIP_MODE_FUSED, tm);
-                    char buf[96];
-                    std::snprintf(buf, sizeof buf, "%s[%d, %d, %d, %.4f]", first ? "" : ", ",
-                                  reg, logt, low, ms);
-                    rep += buf;
-                    first = false;
-                    if (ms < best) {
-                        best = ms;
-                        best_l = logt;
-                        best_c = low;
-                        best_r = reg;
+            // kernel: 0 = LDS-resident tile, 256 / 512 = register tile with
+            // that many threads per workgroup
+            int best_k = 0;
+            int &thr_ref = prec == 8 ? t.rthreads_f64 : t.rthreads_f32;
+            for (int kern : {0, 256, 512}) {
+                for (int logt = 9; logt <= 13; logt++) {
+                    for (int low = 2; low <= 6; low++) {
+                        t.reg_tile = kern != 0;
+                        if (kern) thr_ref = kern;
+                        if (kern && !reg_tile_ok(logt, kern, prec)) continue;
+                        if (!tile_fits(t, logt, low, ab)) continue;
+                        if (prec == 8) {
+                            t.logt_f64 = logt;
+                            t.low_f64 = low;
+                        } else {
+                            t.logt_f32 = logt;
+                            t.low_f32 = low;
+                        }
+                        float ms = time_apply(h, ops, QSLHIP_MODE_FUSED, tm);
+                        char buf[96];
+                        std::snprintf(buf, sizeof buf, "%s[%d, %d, %d, %.4f]", first ? "" : ", ",
+                                      kern, logt, low, ms);
+                        rep += buf;
+                        first = false;
+                        if (ms < best) {
+                            best = ms;
+                            best_l = logt;
+                            best_c = low;
+                            best_k = kern;
+                        }
                     }
                 }
             }
-            t.reg_tile = best_r;
+            t.reg_tile = best_k != 0;
+            thr_ref = best_k ? best_k : 256;
             if (prec == 8) {
                 t.logt_f64 = best_l;
                 t.low_f64 = best_c;
@@ -218,7 +239,7 @@ int qslhip_autotune(int prec, unsigned nqubits, char *report, size_t cap)
                 t.logt_f32 = 11;
                 t.low_f32 = 3;
             }
-            rep += "], \"tile_best\": [" + std::to_string(best_r) + ", " +
+            rep += "], \"tile_best\": [" + std::to_string(best_k) + ", " +
                    std::to_string(best_l) + ", " + std::to_string(best_c) + "]";
             // per-gate threshold: one-op tile vs streaming, target by target
             int thr = -1;

@@ -128,10 +128,12 @@ void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
         else run<float>(h, passes, nullptr, amps, &ops[0]);
         return;
     }
-    bool reg = tuning().reg_tile != 0;
-    for (const TilePass &p : passes) reg = reg && reg_tile_ok(p);
+    const Tuning &tu = tuning();
+    const int threads = h->prec == 8 ? tu.rthreads_f64 : tu.rthreads_f32;
+    bool reg = tu.reg_tile != 0;
+    for (const TilePass &p : passes) reg = reg && reg_tile_ok(p.logt, threads, h->prec);
     if (reg) {
-        launch_reg_passes(h, passes, ops, amps);
+        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups);
         return;
     }
     size_t bytes = ops.size() * sizeof(TileOp);

@@ -4,7 +4,9 @@
 // same operation order (src/qubits/default/gates.cpp); the library is
 // compiled with -ffp-contract=off, so a multiply and an add are never fused
 // and the device results are bit-identical to the reference's -O2 x86-64
-// build (which has no FMA).  Both operate on registers only.
+// build (which has no FMA); the one explicit fma pair (div_sqrt2) computes
+// the correctly rounded quotient the reference's division gives.  All
+// updates operate on registers only.
 #pragma once
 
 #include "internal.h"
@@ -45,14 +47,93 @@ __host__ __device__ constexpr int body_of(int kind)
     }
 }
 
+// x / d for the Fp-rounded d = sqrt(2) of `constexpr Fp sqrt2 =
+// std::sqrt(2)` (gates.cpp:94), correctly rounded like the reference's
+// division, in three operations instead of a full IEEE divide:
+//     q0 = x * r,  e = fma(q0, d, -x) = q0*d - x  (exact),  q = fma(-e, r, q0),
+// with r = RN(1/d).  Markstein's theorem (r within half an ulp of 1/d, q0
+// within one ulp of x/d, no underflow in the residual) makes q = RN(x/d).
+// The residual's sign convention keeps the sign of zero: x = -0 gives
+// q0 = -0, e = +0, q = -0 + -0 = -0.  Tiny nonzero (|x| < 2^-100 FP32,
+// 2^-900 FP64), huge, infinite and NaN x take the division itself.
+// Checked on the CPU against x / d: every FP32 x, and 2^26 sampled FP64 x,
+// bit for bit.
+template <typename T>
+__host__ __device__ __forceinline__ bool div_sqrt2_fast_ok(T ax)
+{
+    constexpr bool f64 = sizeof(T) == 8;
+    const T lo = f64 ? (T)0x1p-900 : (T)0x1p-100f, hi = f64 ? (T)0x1p+900 : (T)0x1p+100f;
+    return (ax <= hi) & ((ax >= lo) | (ax == (T)0));
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ T div_sqrt2_fast(T x)
+{
+    constexpr bool f64 = sizeof(T) == 8;
+    const T d = f64 ? (T)0x1.6a09e667f3bcdp+0 : (T)0x1.6a09e6p+0f;
+    const T r = f64 ? (T)0x1.6a09e667f3bccp-1 : (T)0x1.6a09e6p-1f;
+    const T q0 = x * r;
+    if constexpr (f64) return __builtin_fma(-__builtin_fma(q0, d, -x), r, q0);
+    else return __builtin_fmaf(-__builtin_fmaf(q0, d, -x), r, q0);
+}
+
+// Keeps the rarely taken division branch a branch: an empty volatile asm
+// on the operand cannot be speculated, so the compiler does not turn the
+// branch into a select that evaluates the full IEEE divide on every lane.
+template <typename T>
+__host__ __device__ __forceinline__ T slow_path_operand(T x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(x));
+#endif
+    return x;
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ T div_sqrt2(T x)
+{
+    const T d = sizeof(T) == 8 ? (T)0x1.6a09e667f3bcdp+0 : (T)0x1.6a09e6p+0f;
+    T ax;
+    if constexpr (sizeof(T) == 8) ax = __builtin_fabs(x);
+    else ax = __builtin_fabsf(x);
+    T q = div_sqrt2_fast<T>(x);
+    if (!div_sqrt2_fast_ok<T>(ax)) q = slow_path_operand(x) / d;
+    return q;
+}
+
+// The four quotients of one Hadamard pair with a single range test: the
+// fast path for all four, and the division itself for all four when any
+// of them is outside the safe range (one branch per pair, rarely taken).
+template <typename T>
+__host__ __device__ __forceinline__ void div_sqrt2_x4(T &x0, T &x1, T &x2, T &x3)
+{
+    const T d = sizeof(T) == 8 ? (T)0x1.6a09e667f3bcdp+0 : (T)0x1.6a09e6p+0f;
+    T *xs[4] = {&x0, &x1, &x2, &x3};
+    T q[4];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const T x = *xs[j];
+        T ax;
+        if constexpr (sizeof(T) == 8) ax = __builtin_fabs(x);
+        else ax = __builtin_fabsf(x);
+        ok = ok & div_sqrt2_fast_ok<T>(ax);
+        q[j] = div_sqrt2_fast<T>(x);
+    }
+    if (!ok) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) q[j] = slow_path_operand(*xs[j]) / d;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) *xs[j] = q[j];
+}
+
 // a = amplitude with the acted-on bit 0 (|01> for swap-like gates),
-// b = amplitude with it 1.  `sqrt2` is the Fp-rounded double sqrt(2) of
-// `constexpr Fp sqrt2 = std::sqrt(2)` (gates.cpp:94).
+// b = amplitude with it 1.
 template <typename T>
 __device__ __forceinline__ void pair_update(int body, cplx<T> &a, cplx<T> &b,
                                             T c, T s)
 {
-    const T sqrt2 = (T)1.4142135623730950488016887242097;
     T a0r = a.re, a0i = a.im, a1r = b.re, a1i = b.im;
     switch (body) {
     case QSLHIP_PAULI_X:  // gates.cpp:32-44
@@ -62,9 +143,12 @@ __device__ __forceinline__ void pair_update(int body, cplx<T> &a, cplx<T> &b,
         a.re = a1i; a.im = -a1r; b.re = -a0i; b.im = a0r;
         break;
     case QSLHIP_HADAMARD:  // gates.cpp:83-101 (division, not 1/sqrt2 product)
-        a.re = (a0r + a1r) / sqrt2; a.im = (a0i + a1i) / sqrt2;
-        b.re = (a0r - a1r) / sqrt2; b.im = (a0i - a1i) / sqrt2;
+    {
+        T x0 = a0r + a1r, x1 = a0i + a1i, x2 = a0r - a1r, x3 = a0i - a1i;
+        div_sqrt2_x4<T>(x0, x1, x2, x3);
+        a.re = x0; a.im = x1; b.re = x2; b.im = x3;
         break;
+    }
     case QSLHIP_ROT_X:  // gates.cpp:120-148
         a.re = a0r * c + a1i * s; a.im = a0i * c - a1r * s;
         b.re = a1r * c + a0i * s; b.im = a1i * c - a0r * s;

@@ -119,11 +119,12 @@ namespace qslhip {
 void launch_gate(qslhip_state *h, const GateDesc &g, void *amps = nullptr);
 void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                         const std::vector<TileOp> &ops, void *amps = nullptr);
-// the same passes with the amplitudes held in registers (regtile.hip);
-// every pass must have 10..12 tile qubits
+// the same passes with the amplitudes held in registers, `threads` (256 or
+// 512) per workgroup (regtile.hip); reg_tile_ok says whether a kernel
+// exists for the tile width
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                       const std::vector<TileOp> &ops, void *amps);
-bool reg_tile_ok(const TilePass &p);
+                       const std::vector<TileOp> &ops, void *amps, int threads, int gpw);
+bool reg_tile_ok(int logt, int threads, int prec);
 void launch_fill_basis(qslhip_state *h, uint64_t index);
 double device_prob(qslhip_state *h, unsigned t, unsigned outcome);
 double device_norm2(qslhip_state *h);
@@ -155,6 +156,9 @@ int guarded_call(const std::function<void()> &f);
 void plan_tile_passes(unsigned n, int prec, const std::vector<GateDesc> &gates,
                       int logt, int low_bits, std::vector<TilePass> &passes,
                       std::vector<TileOp> &ops);
+void plan_tile_passes_reorder(unsigned n, const std::vector<GateDesc> &gates, int logt,
+                              int low_bits, std::vector<TilePass> &passes,
+                              std::vector<TileOp> &ops, std::vector<uint32_t> &order);
 GateDesc make_desc(int prec, int kind, unsigned q0, unsigned q1, double angle);
 bool is_gate1(int kind);
 bool is_gate2(int kind);
@@ -162,12 +166,15 @@ bool is_np_kind(int kind);
 
 // Tunables picked by the autotuner (autotune.cpp), with safe defaults.
 struct Tuning {
-    int logt_f64 = 10, low_f64 = 3;
+    int logt_f64 = 11, low_f64 = 3;
     int logt_f32 = 11, low_f32 = 3;
     int lds_max_target_f64 = 3;  // per-gate path: targets below use a tile
     int lds_max_target_f32 = 4;
     int dense_f = 5;             // qubits per fused dense unitary (4 or 5)
-    int reg_tile = 0;            // fused passes hold amplitudes in registers
+    int reg_tile = 1;            // fused passes hold amplitudes in registers
+    int rthreads_f64 = 256;      // workgroup size of the register-tile kernel
+    int rthreads_f32 = 256;
+    int rgroups = 4;             // groups per register-tile workgroup (0: persistent)
 };
 Tuning &tuning();
 

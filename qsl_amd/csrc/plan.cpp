@@ -3,6 +3,7 @@
 // LDS tile passes.
 #include <cmath>
 #include <cstring>
+#include <set>
 
 #include "gate_math.h"
 
@@ -96,6 +97,21 @@ static int tile_pos(uint64_t mask, unsigned q)
     return __builtin_popcountll(mask & ((1ull << q) - 1));
 }
 
+// The TileOp of gate g in a pass whose tile qubits are `mask`.
+static TileOp tile_op_of(const GateDesc &g, uint64_t mask)
+{
+    TileOp op;
+    op.kind = g.kind;
+    op.q0 = g.q0;
+    op.q1 = is_gate2(g.kind) ? g.q1 : 0;
+    op.pa = tile_pos(mask, g.q0);
+    op.pb = is_gate2(g.kind) ? tile_pos(mask, g.q1) : -1;
+    op.pad = 0;
+    op.c = g.c;
+    op.s = g.s;
+    return op;
+}
+
 // Greedy packing in program order: a gate joins the open pass while the
 // union of tile qubits stays within K; otherwise the pass is closed,
 // topped up to K qubits with the lowest free ones (longer contiguous runs
@@ -121,19 +137,7 @@ void plan_tile_passes(unsigned n, int prec, const std::vector<GateDesc> &gates,
         p.logt = K;
         p.tile_mask = m;
         p.op_first = (uint32_t)ops.size();
-        for (size_t i = first; i < end; i++) {
-            const GateDesc &g = gates[i];
-            TileOp op;
-            op.kind = g.kind;
-            op.q0 = g.q0;
-            op.q1 = is_gate2(g.kind) ? g.q1 : 0;
-            op.pa = tile_pos(m, g.q0);
-            op.pb = is_gate2(g.kind) ? tile_pos(m, g.q1) : -1;
-            op.pad = 0;
-            op.c = g.c;
-            op.s = g.s;
-            ops.push_back(op);
-        }
+        for (size_t i = first; i < end; i++) ops.push_back(tile_op_of(gates[i], m));
         p.op_count = (uint32_t)(ops.size() - p.op_first);
         passes.push_back(p);
     };
@@ -149,6 +153,99 @@ void plan_tile_passes(unsigned n, int prec, const std::vector<GateDesc> &gates,
         mask = base | need;
     }
     close(gates.size());
+}
+
+}  // namespace qslhip
+
+namespace qslhip {
+
+// Commutation-aware packing (QSLHIP_MODE_REORDER).  Two gates commute, and
+// may run in either order, when they act on disjoint qubits or are both
+// diagonal in the computational basis (pauliZ, phase, rotateZ, controlZ,
+// controlPhase, controlRotateZ).  Every other pair keeps program order: a
+// gate depends on each earlier gate sharing a qubit, back to the last
+// non-diagonal gate on that qubit.  A pass takes, in index order, every
+// ready gate whose non-diagonal targets are already tile qubits; when none
+// is left it grows the tile by the earliest ready gate that still fits, and
+// it closes when no ready gate fits.  The result equals the program-order
+// product in exact arithmetic; in floating point the amplitudes differ by
+// rounding only.  `order` receives the gate index of every TileOp.
+void plan_tile_passes_reorder(unsigned n, const std::vector<GateDesc> &gates, int logt,
+                              int low_bits, std::vector<TilePass> &passes,
+                              std::vector<TileOp> &ops, std::vector<uint32_t> &order)
+{
+    const int K = logt < (int)n ? logt : (int)n;
+    int c = low_bits < K ? low_bits : K;
+    if (K - c > 8) c = K - 8;
+    const uint64_t base = (c >= 64) ? ~0ull : ((1ull << c) - 1);
+    const size_t m = gates.size();
+    std::vector<uint64_t> need(m);
+    std::vector<std::vector<uint32_t>> succ(m);
+    std::vector<uint32_t> npred(m, 0), stamp(m, 0);
+    std::vector<std::vector<std::pair<uint32_t, bool>>> last(n);
+    for (size_t i = 0; i < m; i++) {
+        const GateDesc &g = gates[i];
+        need[i] = needs_local(g);
+        const bool diag = need[i] == 0;
+        const unsigned qs[2] = {g.q0, g.q1};
+        const int nq = is_gate2(g.kind) ? 2 : 1;
+        for (int k = 0; k < nq; k++) {
+            auto &lq = last[qs[k]];
+            for (const auto &e : lq) {
+                if (diag && e.second) continue;  // diagonal gates commute
+                if (stamp[e.first] == i + 1) continue;
+                stamp[e.first] = (uint32_t)(i + 1);
+                succ[e.first].push_back((uint32_t)i);
+                npred[i]++;
+            }
+            if (diag) lq.emplace_back((uint32_t)i, true);
+            else lq.assign(1, {(uint32_t)i, false});
+        }
+    }
+    std::set<uint32_t> ready;
+    for (size_t i = 0; i < m; i++)
+        if (!npred[i]) ready.insert((uint32_t)i);
+    size_t remaining = m;
+    while (remaining) {
+        uint64_t mask = base;
+        std::vector<uint32_t> take;
+        for (;;) {
+            for (auto it = ready.begin(); it != ready.end();) {
+                const uint32_t i = *it;
+                if (need[i] & ~mask) {
+                    ++it;
+                    continue;
+                }
+                it = ready.erase(it);
+                take.push_back(i);
+                remaining--;
+                for (uint32_t s : succ[i])
+                    if (--npred[s] == 0) ready.insert(s);  // s > i: met later in this sweep
+            }
+            bool grew = false;
+            for (uint32_t i : ready) {
+                if (__builtin_popcountll(mask | need[i]) <= K) {
+                    mask |= need[i];
+                    grew = true;
+                    break;
+                }
+            }
+            if (!grew) break;
+        }
+        QH_REQUIRE(!take.empty(), QSLHIP_EINVAL, "tile too narrow for a gate's targets");
+        uint64_t mk = mask;
+        for (unsigned q = 0; q < n && __builtin_popcountll(mk) < K; q++) mk |= 1ull << q;
+        TilePass p;
+        p.logt = K;
+        p.tile_mask = mk;
+        p.op_first = (uint32_t)ops.size();
+        for (uint32_t i : take) {
+            ops.push_back(tile_op_of(gates[i], mk));
+            order.push_back(i);
+        }
+        p.op_count = (uint32_t)take.size();
+        passes.push_back(p);
+    }
 }
 
 }  // namespace qslhip

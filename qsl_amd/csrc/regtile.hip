@@ -1,96 +1,96 @@
 // regtile.hip -- LDS tile passes with the amplitudes held in registers.
 //
-// A workgroup of 256 threads owns one group of 2^K amplitudes (K = 10..12),
-// R = 2^(K-8) per thread.  A "layout" says which r = K-8 tile bits index a
-// thread's registers (the other 8 index the thread):
-//     tile index of register k of thread t = dep(t, ~rho) | dep(k, rho).
-// Diagonal gates, and the controls of controlled gates, only read the bits
-// of each amplitude's tile index, so they run in registers in any layout.
+// A workgroup of NT = 2^tb threads owns one group of 2^K amplitudes, R =
+// 2^r = 2^(K-tb) per thread.  A "layout" rho (r of the K tile positions)
+// says which tile bits index a thread's registers; the other tb index the
+// thread:  tile index of register k of thread t = dep(t, ~rho) | dep(k, rho).
+//
 // A non-diagonal gate needs its target bit(s) among the register bits; the
-// host planner inserts a remap op (registers -> LDS -> registers in a new
-// layout chosen to cover the next targets) only when they are not.  A pass
-// of 13 gates thus touches LDS a few times instead of 26, and the load and
-// store are straight from/to registers with the thread bits on the lowest
-// tile qubits (coalesced).  The per-amplitude arithmetic is gate_math.h's,
-// in program order, so results are bit-identical to the one-gate kernels.
+// host planner (plan_reg_ops) inserts a remap (registers -> LDS ->
+// registers in a new layout covering the next targets) only when they are
+// not.  Controls and diagonal factors are predicates on single bits, each
+// of which is a register bit (uniform per register: decided on the host,
+// a scalar branch on the device), a thread bit (one lane test per op) or a
+// qubit outside the tile (uniform per workgroup).  The planner decodes every
+// gate into that form, so the device loop does the arithmetic and little
+// else.  Remaps address LDS through an XOR-linear swizzle: the offset of
+// (thread part ^ register part) is the XOR of the two offsets, the register
+// parts come precomputed with the op, and one XOR per element remains.
+//
+// Loads and stores use the layout with registers on the top r positions,
+// so threads sit on the lowest tile qubits (coalesced).  A workgroup may
+// take several groups and prefetch the next while computing one (k_rtile).  The per-amplitude
+// arithmetic is gate_math.h's in program order: results are bit-identical
+// to the one-gate kernels.
 #include <cstring>
 
 #include "tile_ops.h"
 
 namespace qslhip {
 
-// One op of a register-tile pass (host: plan_reg_ops).
+enum RegCode : int32_t { RC_REMAP = 0, RC_PAIR = 1, RC_SWAP = 2, RC_ONE = 3, RC_RZ = 4 };
+
+constexpr int kRegMaxR = 16;
+
+// One decoded op of a register-tile pass.
 struct RegOp {
-    TileOp t;        // t.kind < 0: remap to layout `rho`
-    int32_t ia, ib;  // register-bit indices of the non-diagonal target(s)
-    uint32_t rho;    // remap: tile-bit mask of the new register bits
-    uint32_t pad;
+    int32_t code;    // RegCode
+    int32_t body;    // gate_math body (RC_PAIR, RC_SWAP, RC_ONE)
+    int32_t ia, ib;  // RC_PAIR: register bit of the target; RC_SWAP: of q0 and q1
+    uint32_t kmask;  // registers taking part (RC_PAIR: the bit-ia-clear one of each pair)
+    uint32_t kneg;   // RC_RZ: registers whose target bit is 1
+    uint32_t tmask;  // thread predicate (jt & tmask) == tmask
+    uint32_t tneg;   // RC_RZ: thread-bit target, as a tile-position mask
+    uint64_t gmask;  // group predicate (gbase & gmask) == gmask
+    uint64_t gneg;   // RC_RZ: target qubit outside the tile, as a mask
+    double c, s;
+    uint32_t rho;    // RC_REMAP: register positions of the new layout
+    uint32_t fswap;  // RC_SWAP: also negate |11>
+    uint32_t koff[kRegMaxR];  // RC_REMAP: swizzled LDS byte offset of register k
 };
+
+// LDS element order of the remap buffer.  A 128-byte bank sweep holds
+// B = 128 / A amplitudes (A bytes each); the slot of element e is its low
+// log2(B) bits XORed with the fold of its higher bits, so lanes whose
+// elements differ only above the slot bits still land in distinct banks.
+// The map is linear over XOR: swz(a ^ b) = swz(a) ^ swz(b).
+template <int A>
+__host__ __device__ constexpr uint32_t lds_swz(uint32_t e)
+{
+    constexpr uint32_t B = 128 / A;
+    constexpr uint32_t L = B == 8 ? 3 : 4;
+    const uint32_t x = e >> L;
+    return e ^ ((x ^ (x >> L) ^ (x >> (2 * L))) & (B - 1));
+}
 
 namespace {
 
-constexpr int kBlock = 256;
-
-// LDS element order of the remap buffer.  A 128-byte bank sweep holds
-// B = 128 / sizeof(cplx<T>) elements (8 FP64 or 16 FP32 amplitudes); the
-// slot of element e is its low log2(B) bits XORed with the fold of its
-// higher bits, so lanes whose elements differ only above the slot bits
-// (register bits among the low tile qubits) still land in distinct banks.
-template <typename T>
-struct LdsSwizzle {
-    static constexpr uint32_t kB = 128 / (uint32_t)sizeof(cplx<T>);
-    static constexpr uint32_t kL = kB == 8 ? 3 : 4;
-    __device__ __forceinline__ uint32_t operator()(uint32_t e) const
-    {
-        const uint32_t x = e >> kL;
-        return e ^ ((x ^ (x >> kL) ^ (x >> (2 * kL))) & (kB - 1));
-    }
-};
-
-// Where the tile of one workgroup lives: the tile index j = t | k << 8 of
-// the load/store layout (thread t on the 8 lowest tile qubits, register k
-// on the top r) maps to the state index gbase | dep(t) | kpart(k).
+// Where the tile of one workgroup lives: the tile index j = t | k << tb of
+// the load/store layout maps to the state index gbase | dep(t) | kpart(k).
 struct RegTileArgs {
-    uint64_t tile_mask;  // the K tile qubits
-    uint64_t tlow;       // the 8 lowest of them (thread bits)
-    uint64_t kq[4];      // qubit of tile position 8 + i, as a one-bit mask
+    uint64_t tile_mask;          // the K tile qubits
+    uint64_t tlow;               // the tb lowest of them (thread bits)
+    uint64_t kq[4];              // qubit of tile position tb + i, as a one-bit mask
+    uint32_t koff0[kRegMaxR];    // swizzled LDS byte offsets of the load layout
 };
 
-__device__ __forceinline__ uint32_t dep32(uint32_t v, uint32_t mask)
-{
-    uint32_t r = 0;
-    while (mask) {
-        uint32_t low = mask & (0u - mask);
-        if (v & 1u) r |= low;
-        v >>= 1;
-        mask ^= low;
-    }
-    return r;
-}
-
-// tile index of every register of this thread in layout rho
-template <int R>
-__device__ __forceinline__ void layout_index(uint32_t (&jk)[R], uint32_t rho, uint32_t kmask)
-{
-    const uint32_t tb = dep32(threadIdx.x, kmask & ~rho);
-#pragma unroll
-    for (int k = 0; k < R; k++) jk[k] = tb | dep32((uint32_t)k, rho);
-}
-
-// rotateZ's two diagonal halves without a branch: the |1> half is the |0>
-// half with s negated (x + y*(-s) is x - y*s exactly in IEEE arithmetic),
-// so the result is bit-identical to gate_math.h's rz_half.
+// rotateZ's two diagonal halves without a branch on the bit: the |1> half
+// is the |0> half with s negated (x + y*(-s) is x - y*s exactly in IEEE
+// arithmetic), so the result is bit-identical to gate_math.h's rz_half.
 template <typename T>
-__device__ __forceinline__ void rz_sel(bool one, cplx<T> &x, T c, T s)
+__device__ __forceinline__ void rz_s(cplx<T> &x, T c, T ss)
 {
-    const T ss = one ? -s : s;
     const T r = x.re, i = x.im;
     x.re = r * c + i * ss;
     x.im = i * c - r * ss;
 }
 
-// Pair op with the target on register bit I: registers k and k | 2^I; the
-// control (a tile bit cpos >= 0, or none) is evaluated per element.
+template <typename T>
+__device__ __forceinline__ cplx<T> pick(bool p, const cplx<T> &a, const cplx<T> &b)
+{
+    return cplx<T>{p ? a.re : b.re, p ? a.im : b.im};
+}
+
 // The register array must stay in registers: every index into it is a
 // compile-time constant.  The empty asm markers (distinct immediates per
 // instantiation) keep the optimiser from merging the cases of a runtime
@@ -98,23 +98,27 @@ __device__ __forceinline__ void rz_sel(bool one, cplx<T> &x, T c, T s)
 // would move the array to scratch memory.
 #define QH_CASE_MARK(A, B) asm volatile("; rtile %0 %1" ::"n"(A), "n"(B))
 
-template <typename T, int R, int I, int BODY>
-__device__ __forceinline__ void reg_pair(cplx<T> (&v)[R], const uint32_t (&jk)[R], int cpos,
-                                         T c, T s)
+// Pair op with the target on register bit I; kmask (uniform) selects the
+// pairs, tp (per lane, SEL only) the lanes whose control is set.
+template <typename T, int R, int I, int BODY, bool SEL>
+__device__ __forceinline__ void reg_pair(cplx<T> (&v)[R], uint32_t kmask, bool tp, T c, T s)
 {
-    QH_CASE_MARK(I, BODY);
+    QH_CASE_MARK(I, BODY + 64 * SEL);
 #pragma unroll
     for (int k = 0; k < R; k++) {
         if (k & (1 << I)) continue;
+        if (!((kmask >> k) & 1u)) continue;
         cplx<T> a = v[k], b = v[k | (1 << I)];
         pair_update<T>(BODY, a, b, c, s);
-        const bool on = cpos < 0 || ((jk[k] >> cpos) & 1u);
-        if (on) {
+        if (SEL) {
+            v[k] = pick(tp, a, v[k]);
+            v[k | (1 << I)] = pick(tp, b, v[k | (1 << I)]);
+        } else {
             v[k] = a;
             v[k | (1 << I)] = b;
         }
     }
-    QH_CASE_MARK(I, BODY);
+    QH_CASE_MARK(I, BODY + 64 * SEL);
 }
 
 // Swap-like op on register bits IA (q0) and IB (q1): |q0=1,q1=0> paired
@@ -132,15 +136,29 @@ __device__ __forceinline__ void reg_swaplike(cplx<T> (&v)[R], T c, T s)
     QH_CASE_MARK(IA * 4 + IB, BODY + 64 * FSWAP);
 }
 
-template <typename T, int R, int BODY>
-__device__ __forceinline__ void dispatch_pair(cplx<T> (&v)[R], const uint32_t (&jk)[R], int ia,
-                                              int cpos, T c, T s)
+template <typename T, int R, int BODY, bool SEL>
+__device__ __forceinline__ void dispatch_pair(cplx<T> (&v)[R], int ia, uint32_t kmask, bool tp,
+                                              T c, T s)
 {
     switch (ia) {
-    case 0: reg_pair<T, R, 0, BODY>(v, jk, cpos, c, s); break;
-    case 1: reg_pair<T, R, 1, BODY>(v, jk, cpos, c, s); break;
-    case 2: if constexpr (R > 4) reg_pair<T, R, 2, BODY>(v, jk, cpos, c, s); break;
-    case 3: if constexpr (R > 8) reg_pair<T, R, 3, BODY>(v, jk, cpos, c, s); break;
+    case 0: reg_pair<T, R, 0, BODY, SEL>(v, kmask, tp, c, s); break;
+    case 1: reg_pair<T, R, 1, BODY, SEL>(v, kmask, tp, c, s); break;
+    case 2: if constexpr (R > 4) reg_pair<T, R, 2, BODY, SEL>(v, kmask, tp, c, s); break;
+    case 3: if constexpr (R > 8) reg_pair<T, R, 3, BODY, SEL>(v, kmask, tp, c, s); break;
+    default: break;
+    }
+}
+
+template <typename T, int R, bool SEL>
+__device__ __forceinline__ void dispatch_body(cplx<T> (&v)[R], int body, int ia, uint32_t kmask,
+                                              bool tp, T c, T s)
+{
+    switch (body) {
+    case QSLHIP_PAULI_X: dispatch_pair<T, R, QSLHIP_PAULI_X, SEL>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_PAULI_Y: dispatch_pair<T, R, QSLHIP_PAULI_Y, SEL>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_HADAMARD: dispatch_pair<T, R, QSLHIP_HADAMARD, SEL>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_ROT_X: dispatch_pair<T, R, QSLHIP_ROT_X, SEL>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_ROT_Y: dispatch_pair<T, R, QSLHIP_ROT_Y, SEL>(v, ia, kmask, tp, c, s); break;
     default: break;
     }
 }
@@ -161,143 +179,193 @@ __device__ __forceinline__ void dispatch_swap(cplx<T> (&v)[R], int ia, int ib, T
 #undef QH_SW
 }
 
-// Apply one gate to the registers.  bit(q) of an amplitude: a tile bit
-// (position >= 0) of its index, or the group's value of a non-tile qubit.
-template <typename T, int R>
-__device__ __forceinline__ void reg_op(cplx<T> (&v)[R], const uint32_t (&jk)[R],
-                                       uint64_t gbase, const RegOp &o)
+template <typename T, int R, int BODY, bool SEL>
+__device__ __forceinline__ void reg_one(cplx<T> (&v)[R], uint32_t kmask, bool tp, T c, T s)
 {
-    const TileOp &op = o.t;
-    const T c = (T)op.c, s = (T)op.s;
-    const int pa = op.pa, pb = op.pb;
-    auto bit = [&](int pos, uint32_t q, uint32_t j) -> bool {
-        return pos >= 0 ? ((j >> pos) & 1u) : ((gbase >> q) & 1ull);
-    };
-    switch (op.kind) {
-    case QSLHIP_PAULI_X: dispatch_pair<T, R, QSLHIP_PAULI_X>(v, jk, o.ia, -1, c, s); break;
-    case QSLHIP_PAULI_Y: dispatch_pair<T, R, QSLHIP_PAULI_Y>(v, jk, o.ia, -1, c, s); break;
-    case QSLHIP_HADAMARD: dispatch_pair<T, R, QSLHIP_HADAMARD>(v, jk, o.ia, -1, c, s); break;
-    case QSLHIP_ROT_X: dispatch_pair<T, R, QSLHIP_ROT_X>(v, jk, o.ia, -1, c, s); break;
-    case QSLHIP_ROT_Y: dispatch_pair<T, R, QSLHIP_ROT_Y>(v, jk, o.ia, -1, c, s); break;
-    case QSLHIP_ROT_Z:
 #pragma unroll
-        for (int k = 0; k < R; k++) rz_sel<T>(bit(pa, op.q0, jk[k]), v[k], c, s);
-        break;
-    case QSLHIP_PAULI_Z:
-    case QSLHIP_PHASE: {
-        const int body = op.kind;
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-            cplx<T> x = v[k];
-            one_update<T>(body, x, c, s);
-            if (bit(pa, op.q0, jk[k])) v[k] = x;
-        }
-        break;
-    }
-    case QSLHIP_CZ:
-    case QSLHIP_CPHASE: {
-        const int body = body_of(op.kind);
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-            cplx<T> x = v[k];
-            one_update<T>(body, x, c, s);
-            if (bit(pa, op.q0, jk[k]) && bit(pb, op.q1, jk[k])) v[k] = x;
-        }
-        break;
-    }
-    case QSLHIP_CROT_Z:
-        if (pa < 0 && !((gbase >> op.q0) & 1ull)) break;
-#pragma unroll
-        for (int k = 0; k < R; k++) {
-            cplx<T> x = v[k];
-            rz_sel<T>(bit(pb, op.q1, jk[k]), x, c, s);
-            if (pa < 0 || ((jk[k] >> pa) & 1u)) v[k] = x;
-        }
-        break;
-    case QSLHIP_CNOT:
-    case QSLHIP_CY:
-    case QSLHIP_CROT_X:
-    case QSLHIP_CROT_Y:
-    case QSLHIP_CHADAMARD: {
-        if (pa < 0 && !((gbase >> op.q0) & 1ull)) break;  // control off for the group
-        const int cpos = pa;
-        switch (body_of(op.kind)) {
-        case QSLHIP_PAULI_X: dispatch_pair<T, R, QSLHIP_PAULI_X>(v, jk, o.ia, cpos, c, s); break;
-        case QSLHIP_PAULI_Y: dispatch_pair<T, R, QSLHIP_PAULI_Y>(v, jk, o.ia, cpos, c, s); break;
-        case QSLHIP_ROT_X: dispatch_pair<T, R, QSLHIP_ROT_X>(v, jk, o.ia, cpos, c, s); break;
-        case QSLHIP_ROT_Y: dispatch_pair<T, R, QSLHIP_ROT_Y>(v, jk, o.ia, cpos, c, s); break;
-        default: dispatch_pair<T, R, QSLHIP_HADAMARD>(v, jk, o.ia, cpos, c, s); break;
-        }
-        break;
-    }
-    case QSLHIP_SWAP:
-        dispatch_swap<T, R, QSLHIP_PAULI_X, false>(v, o.ia, o.ib, c, s);
-        break;
-    case QSLHIP_FSWAP:
-        dispatch_swap<T, R, QSLHIP_PAULI_X, true>(v, o.ia, o.ib, c, s);
-        break;
-    case QSLHIP_NP_ROT_X:
-        dispatch_swap<T, R, QSLHIP_ROT_X, false>(v, o.ia, o.ib, c, s);
-        break;
-    case QSLHIP_NP_ROT_Y:
-        dispatch_swap<T, R, QSLHIP_ROT_Y, false>(v, o.ia, o.ib, c, s);
-        break;
-    case QSLHIP_NP_HADAMARD:
-        dispatch_swap<T, R, QSLHIP_HADAMARD, false>(v, o.ia, o.ib, c, s);
-        break;
-    default:
-        break;
+    for (int k = 0; k < R; k++) {
+        if (!((kmask >> k) & 1u)) continue;
+        cplx<T> x = v[k];
+        one_update<T>(BODY, x, c, s);
+        v[k] = SEL ? pick(tp, x, v[k]) : x;
     }
 }
 
-template <typename T, int R>
-__global__ __launch_bounds__(kBlock) void k_rtile(cplx<T> *__restrict__ st, unsigned n,
-                                                  RegTileArgs ta,
-                                                  const RegOp *__restrict__ ops, int nops)
+// x with its sign flipped when neg (an exact negation: one XOR of the sign
+// bit, with no select between two negated copies).
+__device__ __forceinline__ double flip_sign(double x, bool neg)
+{
+    return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)neg << 63));
+}
+__device__ __forceinline__ float flip_sign(float x, bool neg)
+{
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ ((uint32_t)neg << 31));
+}
+
+// rotateZ (and controlRotateZ): the half of every register follows its
+// target bit: gneg (group), kneg (register k) and the lane's own sl.
+template <typename T, int R, bool SEL>
+__device__ __forceinline__ void reg_rz(cplx<T> (&v)[R], uint32_t kmask, uint32_t kneg, bool gneg,
+                                       bool tp, T c, T sl)
+{
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        if (!((kmask >> k) & 1u)) continue;
+        cplx<T> x = v[k];
+        rz_s<T>(x, c, flip_sign(sl, gneg != (bool)((kneg >> k) & 1u)));
+        v[k] = SEL ? pick(tp, x, v[k]) : x;
+    }
+}
+
+// Workgroup barrier for the LDS remap buffer: waits for this wave's LDS
+// traffic only, so global loads already issued for the next group stay in
+// flight across it (__syncthreads would also drain them).
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// s_waitcnt vmcnt(N) (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt and
+// lgkmcnt left at "no wait"): at most N of this wave's vector memory
+// operations still outstanding.  The compiler's wait insertion sees it and
+// needs no waits of its own for anything older.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Workgroup b takes groups b, b + gridDim.x, ... (one group when the grid
+// covers them all; persistent workgroups when it is the device's resident
+// count) and loads its next group into a second register set while it
+// computes and stores the current one, so HBM traffic overlaps the gate
+// arithmetic.
+template <typename T, int NT, int R>
+__global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned n,
+                                              RegTileArgs ta, const RegOp *__restrict__ ops,
+                                              int nops, uint64_t ngroups)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem);
-    const LdsSwizzle<T> sw;
+    constexpr int A = (int)sizeof(cplx<T>);
     constexpr int r = R == 4 ? 2 : R == 8 ? 3 : 4;
-    constexpr int K = 8 + r;
+    const uint32_t t = threadIdx.x;
     const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1);
-    const uint64_t gbase = deposit((uint64_t)blockIdx.x, full & ~ta.tile_mask);
-    const uint64_t tpart = gbase | deposit((uint64_t)threadIdx.x, ta.tlow);
-    uint64_t addr[R];
+    const uint64_t outside = full & ~ta.tile_mask;
+    const uint64_t tthread = deposit((uint64_t)t, ta.tlow);
+    uint64_t kpart[R];
 #pragma unroll
     for (int k = 0; k < R; k++) {
         uint64_t kp = 0;
 #pragma unroll
         for (int i = 0; i < r; i++)
             if ((k >> i) & 1) kp |= ta.kq[i];
-        addr[k] = tpart | kp;
+        kpart[k] = kp;
     }
-    const uint32_t kmask = (1u << K) - 1;
-    const uint32_t rho0 = kmask & ~0xFFu;  // registers on the top tile bits
-    uint32_t jk[R];
-    layout_index<R>(jk, rho0, kmask);
-    cplx<T> v[R];
+    uint64_t g = blockIdx.x;
+    if (g >= ngroups) return;
+    uint64_t gbase_n = deposit(g, outside);
+    cplx<T> v[R], vn[R];
 #pragma unroll
-    for (int k = 0; k < R; k++) v[k] = st[addr[k]];
-    for (int i = 0; i < nops; i++) {
-        const RegOp o = ops[i];
-        if (o.t.kind < 0) {
+    for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];
+    wait_vmcnt<0>();
+    for (;;) {
+        const uint64_t gbase = gbase_n;
 #pragma unroll
-            for (int k = 0; k < R; k++) lds[sw(jk[k])] = v[k];
-            __syncthreads();
-            layout_index<R>(jk, o.rho, kmask);
+        for (int k = 0; k < R; k++) v[k] = vn[k];
+        const uint64_t gn = g + gridDim.x;
+        const bool more = gn < ngroups;
+        if (more) {
+            gbase_n = deposit(gn, outside);
 #pragma unroll
-            for (int k = 0; k < R; k++) v[k] = lds[sw(jk[k])];
-            __syncthreads();
+            for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];
+            // in flight: the previous group's R stores and these R loads;
+            // everything older (this group's loads) is complete
+            wait_vmcnt<2 * R>();
         } else {
-            reg_op<T, R>(v, jk, gbase, o);
+            wait_vmcnt<R>();  // only the previous group's stores may remain
         }
-    }
+        uint32_t jt = t;  // this thread's tile-index bits (zeros at register positions)
+        uint32_t base = lds_swz<A>(jt) * A;
+        uint32_t koff[R];
 #pragma unroll
-    for (int k = 0; k < R; k++) st[addr[k]] = v[k];
+        for (int k = 0; k < R; k++) koff[k] = ta.koff0[k];
+        for (int i = 0; i < nops; i++) {
+            const RegOp &o = ops[i];
+            const int code = o.code;
+            if (code == RC_REMAP) {
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                    *reinterpret_cast<cplx<T> *>(smem + (base ^ koff[k])) = v[k];
+                lds_barrier();
+                jt = t;
+                for (uint32_t rho = o.rho; rho; rho &= rho - 1) {  // insert zeros at rho
+                    const uint32_t lowm = (rho & (0u - rho)) - 1;
+                    jt = (jt & lowm) | ((jt & ~lowm) << 1);
+                }
+                base = lds_swz<A>(jt) * A;
+#pragma unroll
+                for (int k = 0; k < R; k++) koff[k] = o.koff[k];
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                    v[k] = *reinterpret_cast<const cplx<T> *>(smem + (base ^ koff[k]));
+                lds_barrier();
+                continue;
+            }
+            if ((gbase & o.gmask) != o.gmask) continue;  // a control outside the tile is 0
+            const T c = (T)o.c, s = (T)o.s;
+            const uint32_t tmask = o.tmask;
+            const bool sel = tmask != 0;
+            const bool tp = (jt & tmask) == tmask;
+            const uint32_t kmask = o.kmask;
+            switch (code) {
+            case RC_PAIR:
+                if (sel) dispatch_body<T, R, true>(v, o.body, o.ia, kmask, tp, c, s);
+                else dispatch_body<T, R, false>(v, o.body, o.ia, kmask, tp, c, s);
+                break;
+            case RC_SWAP:
+                switch (o.body) {
+                case QSLHIP_PAULI_X:
+                    if (o.fswap) dispatch_swap<T, R, QSLHIP_PAULI_X, true>(v, o.ia, o.ib, c, s);
+                    else dispatch_swap<T, R, QSLHIP_PAULI_X, false>(v, o.ia, o.ib, c, s);
+                    break;
+                case QSLHIP_ROT_X:
+                    dispatch_swap<T, R, QSLHIP_ROT_X, false>(v, o.ia, o.ib, c, s);
+                    break;
+                case QSLHIP_ROT_Y:
+                    dispatch_swap<T, R, QSLHIP_ROT_Y, false>(v, o.ia, o.ib, c, s);
+                    break;
+                default:
+                    dispatch_swap<T, R, QSLHIP_HADAMARD, false>(v, o.ia, o.ib, c, s);
+                    break;
+                }
+                break;
+            case RC_ONE:
+                if (o.body == QSLHIP_PAULI_Z) {
+                    if (sel) reg_one<T, R, QSLHIP_PAULI_Z, true>(v, kmask, tp, c, s);
+                    else reg_one<T, R, QSLHIP_PAULI_Z, false>(v, kmask, tp, c, s);
+                } else {
+                    if (sel) reg_one<T, R, QSLHIP_PHASE, true>(v, kmask, tp, c, s);
+                    else reg_one<T, R, QSLHIP_PHASE, false>(v, kmask, tp, c, s);
+                }
+                break;
+            default: {  // RC_RZ
+                const bool gneg = (gbase & o.gneg) != 0;
+                const T sl = flip_sign(s, (jt & o.tneg) != 0);
+                if (sel) reg_rz<T, R, true>(v, kmask, o.kneg, gneg, tp, c, sl);
+                else reg_rz<T, R, false>(v, kmask, o.kneg, gneg, tp, c, sl);
+                break;
+            }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) st[gbase | tthread | kpart[k]] = v[k];
+        if (!more) break;
+        g = gn;
+    }
 }
 
-// Tile qubits a non-diagonal body moves amplitudes across (tile positions).
+// Tile positions a non-diagonal body moves amplitudes across.
 void targets_of(const TileOp &t, int out[2], int &cnt)
 {
     cnt = 0;
@@ -310,24 +378,64 @@ void targets_of(const TileOp &t, int out[2], int &cnt)
     }
 }
 
+uint32_t dep_host(uint32_t v, uint32_t mask)
+{
+    uint32_t out = 0;
+    for (int b = 0; mask; b++, mask &= mask - 1)
+        if ((v >> b) & 1u) out |= mask & (0u - mask);
+    return out;
+}
+
+uint32_t swz_bytes(uint32_t e, int A)
+{
+    return A == 16 ? lds_swz<16>(e) * 16 : lds_swz<8>(e) * 8;
+}
+
 }  // namespace
 
-// Host planner: the register layouts of one pass.  Starts and ends in the
-// load/store layout (registers on the top r tile bits); a remap is inserted
-// before a gate whose targets are not register bits, choosing the gate's
-// targets plus the targets of the gates that follow, in order.
-void plan_reg_ops(const TileOp *ops, uint32_t count, int K, std::vector<RegOp> &out)
+// Host planner: decode the TileOps of one pass (tile width K, 2^tb
+// threads, A bytes per amplitude) into register ops.  The pass starts and
+// ends in the load/store layout (registers on the top r positions); a remap
+// goes before a gate whose targets are not register bits, to the gate's
+// targets plus the next gates' targets in order (the set that postpones the
+// following remap longest), topped up from the current layout.
+void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
+                  std::vector<RegOp> &out)
 {
-    const int r = K - 8;
-    const uint32_t kmask = (1u << K) - 1;
-    const uint32_t rho0 = kmask & ~0xFFu;
+    const int r = K - tb, R = 1 << r;
+    const uint32_t all_k = (R >= 32) ? ~0u : ((1u << R) - 1);
+    const uint32_t rho0 = ((1u << K) - 1) & ~((1u << tb) - 1);
     uint32_t cur = rho0;
-    auto slot = [](uint32_t rho, int pos) {
-        return __builtin_popcount(rho & ((1u << pos) - 1));
+    auto slot = [&](int pos) { return __builtin_popcount(cur & ((1u << pos) - 1)); };
+    auto kset = [&](int pos) {  // registers whose bit at tile position pos is 1
+        const int sl = slot(pos);
+        uint32_t m = 0;
+        for (int k = 0; k < R; k++)
+            if ((k >> sl) & 1) m |= 1u << k;
+        return m;
+    };
+    auto remap = [&](uint32_t rho) {
+        RegOp m{};
+        m.code = RC_REMAP;
+        m.rho = rho;
+        for (int k = 0; k < R; k++) m.koff[k] = swz_bytes(dep_host((uint32_t)k, rho), A);
+        out.push_back(m);
+        cur = rho;
+    };
+    auto pred = [&](RegOp &o, int pos, uint32_t q) {  // require qubit q (tile pos) == 1
+        if (pos < 0) o.gmask |= 1ull << q;
+        else if ((cur >> pos) & 1u) o.kmask &= kset(pos);
+        else o.tmask |= 1u << pos;
+    };
+    auto neg = [&](RegOp &o, int pos, uint32_t q) {  // rotateZ half follows qubit q
+        if (pos < 0) o.gneg = 1ull << q;
+        else if ((cur >> pos) & 1u) o.kneg = kset(pos);
+        else o.tneg = 1u << pos;
     };
     for (uint32_t i = 0; i < count; i++) {
+        const TileOp &t = ops[i];
         int tg[2], nt;
-        targets_of(ops[i], tg, nt);
+        targets_of(t, tg, nt);
         uint32_t need = 0;
         for (int k = 0; k < nt; k++) need |= 1u << tg[k];
         if ((need & ~cur) != 0) {
@@ -338,77 +446,147 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, std::vector<RegOp> &
                 for (int k = 0; k < nj && __builtin_popcount(rho) < r; k++) rho |= 1u << tj[k];
             }
             for (int p = K - 1; p >= 0 && __builtin_popcount(rho) < r; p--)
-                if (cur & (1u << p)) rho |= 1u << p;
+                if ((cur >> p) & 1u) rho |= 1u << p;
             for (int p = K - 1; p >= 0 && __builtin_popcount(rho) < r; p--) rho |= 1u << p;
-            RegOp m{};
-            m.t.kind = -1;
-            m.rho = rho;
-            out.push_back(m);
-            cur = rho;
+            remap(rho);
         }
         RegOp o{};
-        o.t = ops[i];
-        o.ia = nt > 0 ? slot(cur, tg[0]) : 0;
-        o.ib = nt > 1 ? slot(cur, tg[1]) : 0;
+        o.c = t.c;
+        o.s = t.s;
+        o.kmask = all_k;
+        const int body = body_of(t.kind);
+        switch (t.kind) {
+        case QSLHIP_PAULI_X: case QSLHIP_PAULI_Y: case QSLHIP_HADAMARD:
+        case QSLHIP_ROT_X: case QSLHIP_ROT_Y:
+            o.code = RC_PAIR;
+            o.body = body;
+            o.ia = slot(t.pa);
+            o.kmask = all_k & ~kset(t.pa);
+            break;
+        case QSLHIP_ROT_Z:
+            o.code = RC_RZ;
+            neg(o, t.pa, t.q0);
+            break;
+        case QSLHIP_PAULI_Z: case QSLHIP_PHASE:
+            o.code = RC_ONE;
+            o.body = body;
+            pred(o, t.pa, t.q0);
+            break;
+        case QSLHIP_CNOT: case QSLHIP_CY: case QSLHIP_CROT_X: case QSLHIP_CROT_Y:
+        case QSLHIP_CHADAMARD:
+            o.code = RC_PAIR;
+            o.body = body;
+            o.ia = slot(t.pb);
+            o.kmask = all_k & ~kset(t.pb);
+            pred(o, t.pa, t.q0);
+            break;
+        case QSLHIP_CROT_Z:
+            o.code = RC_RZ;
+            pred(o, t.pa, t.q0);
+            neg(o, t.pb, t.q1);
+            break;
+        case QSLHIP_CZ: case QSLHIP_CPHASE:
+            o.code = RC_ONE;
+            o.body = body;
+            pred(o, t.pa, t.q0);
+            pred(o, t.pb, t.q1);
+            break;
+        default:  // swap-like
+            o.code = RC_SWAP;
+            o.body = body;
+            o.ia = slot(t.pa);
+            o.ib = slot(t.pb);
+            o.fswap = t.kind == QSLHIP_FSWAP;
+            break;
+        }
         out.push_back(o);
     }
-    if (cur != rho0) {
-        RegOp m{};
-        m.t.kind = -1;
-        m.rho = rho0;
-        out.push_back(m);
-    }
+    if (cur != rho0) remap(rho0);
 }
 
-template <typename T>
-static void launch_rtile_t(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dops,
-                           uint32_t nops)
+namespace {
+
+template <typename T, int NT, int R>
+void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dops, uint32_t nops,
+                int gpw)
 {
-    const int K = p.logt;
+    constexpr int A = (int)sizeof(cplx<T>);
+    constexpr int tb = NT == 256 ? 8 : 9;
+    constexpr int r = R == 4 ? 2 : R == 8 ? 3 : 4;
+    const int K = tb + r;
     RegTileArgs ta{};
     ta.tile_mask = p.tile_mask;
     int pos = 0;
     for (unsigned q = 0; q < 64; q++) {
         if (!((p.tile_mask >> q) & 1ull)) continue;
-        if (pos < 8) ta.tlow |= 1ull << q;
-        else ta.kq[pos - 8] = 1ull << q;
+        if (pos < tb) ta.tlow |= 1ull << q;
+        else ta.kq[pos - tb] = 1ull << q;
         pos++;
     }
-    QH_REQUIRE(pos == K, QSLHIP_EINVAL, "tile mask does not hold the tile width");
-    const size_t shmem = sizeof(cplx<T>) << K;
+    QH_REQUIRE(pos == K && p.logt == K, QSLHIP_EINVAL, "tile mask does not hold the tile width");
+    for (int k = 0; k < R; k++) ta.koff0[k] = swz_bytes((uint32_t)k << tb, A);
+    const size_t shmem = (size_t)A << K;
     QH_REQUIRE(shmem <= 64 * 1024, QSLHIP_EINVAL, "register tile needs more than 64 KiB of LDS");
-    const unsigned groups = (unsigned)(1ull << (h->n - K));
-    cplx<T> *st = static_cast<cplx<T> *>(amps);
-    switch (K) {
-    case 10:
-        hipLaunchKernelGGL((k_rtile<T, 4>), dim3(groups), dim3(kBlock), shmem, h->stream, st,
-                           h->n, ta, dops, (int)nops);
-        break;
-    case 11:
-        hipLaunchKernelGGL((k_rtile<T, 8>), dim3(groups), dim3(kBlock), shmem, h->stream, st,
-                           h->n, ta, dops, (int)nops);
-        break;
-    case 12:
-        hipLaunchKernelGGL((k_rtile<T, 16>), dim3(groups), dim3(kBlock), shmem, h->stream, st,
-                           h->n, ta, dops, (int)nops);
-        break;
-    default:
-        throw Error(QSLHIP_EINVAL, "register tiles take 10..12 tile qubits");
+    QH_REQUIRE(h->n >= (unsigned)K, QSLHIP_EINVAL, "tile wider than the state");
+    const uint64_t groups = 1ull << (h->n - K);
+    // gpw groups per workgroup (0: persistent, as many workgroups as the
+    // device holds at once)
+    uint64_t grid;
+    if (gpw > 0) {
+        grid = (groups + (uint64_t)gpw - 1) / (uint64_t)gpw;
+    } else {
+        static int resident = 0;  // per instantiation
+        if (!resident) {
+            int per_cu = 0, dev = 0, cus = 0;
+            QH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rtile<T, NT, R>,
+                                                                  NT, shmem));
+            QH_CHECK(hipGetDevice(&dev));
+            QH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            resident = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+        }
+        grid = groups < (uint64_t)resident ? groups : (uint64_t)resident;
     }
+    hipLaunchKernelGGL((k_rtile<T, NT, R>), dim3((unsigned)grid), dim3(NT), shmem, h->stream,
+                       static_cast<cplx<T> *>(amps), h->n, ta, dops, (int)nops, groups);
     QH_CHECK(hipGetLastError());
 }
 
-bool reg_tile_ok(const TilePass &p) { return p.logt >= 10 && p.logt <= 12; }
+template <typename T>
+void launch_t(qslhip_state *h, void *amps, const TilePass &p, int nt, const RegOp *dops,
+              uint32_t nops, int gpw)
+{
+    const int r = p.logt - (nt == 512 ? 9 : 8);
+    if (nt == 256 && r == 2) launch_one<T, 256, 4>(h, amps, p, dops, nops, gpw);
+    else if (nt == 256 && r == 3) launch_one<T, 256, 8>(h, amps, p, dops, nops, gpw);
+    else if (nt == 256 && r == 4) launch_one<T, 256, 16>(h, amps, p, dops, nops, gpw);
+    else if (nt == 512 && r == 3) launch_one<T, 512, 8>(h, amps, p, dops, nops, gpw);
+    else if (nt == 512 && r == 4) launch_one<T, 512, 16>(h, amps, p, dops, nops, gpw);
+    else throw Error(QSLHIP_EINVAL, "no register-tile kernel for this tile width");
+}
+
+}  // namespace
+
+bool reg_tile_ok(int logt, int threads, int prec)
+{
+    const int r = logt - (threads == 512 ? 9 : 8);
+    if (threads != 256 && threads != 512) return false;
+    if (r < 2 || r > 4 || (threads == 512 && r == 2)) return false;
+    return ((size_t)(prec == 8 ? 16 : 8) << logt) <= 64 * 1024;
+}
 
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                       const std::vector<TileOp> &ops, void *amps)
+                       const std::vector<TileOp> &ops, void *amps, int threads, int gpw)
 {
+    const int tb = threads == 512 ? 9 : 8;
+    const int A = h->prec == 8 ? 16 : 8;
     std::vector<RegOp> rops;
     std::vector<uint32_t> first(passes.size() + 1);
     for (size_t i = 0; i < passes.size(); i++) {
-        QH_REQUIRE(reg_tile_ok(passes[i]), QSLHIP_EINVAL, "register tiles take 10..12 tile qubits");
+        QH_REQUIRE(reg_tile_ok(passes[i].logt, threads, h->prec), QSLHIP_EINVAL,
+                   "no register-tile kernel for this tile width");
         first[i] = (uint32_t)rops.size();
-        plan_reg_ops(ops.data() + passes[i].op_first, passes[i].op_count, passes[i].logt, rops);
+        plan_reg_ops(ops.data() + passes[i].op_first, passes[i].op_count, passes[i].logt, tb, A,
+                     rops);
     }
     first[passes.size()] = (uint32_t)rops.size();
     const size_t bytes = rops.size() * sizeof(RegOp);
@@ -421,11 +599,11 @@ void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
     QH_CHECK(hipEventRecord(h->staged, h->stream));
     const RegOp *dops = static_cast<const RegOp *>(h->optab.ptr);
     for (size_t i = 0; i < passes.size(); i++) {
-        const TilePass &p = passes[i];
-        QH_REQUIRE(h->n >= (unsigned)p.logt, QSLHIP_EINVAL, "tile wider than the state");
         const uint32_t n_ops = first[i + 1] - first[i];
-        if (h->prec == 8) launch_rtile_t<double>(h, amps, p, dops + first[i], n_ops);
-        else launch_rtile_t<float>(h, amps, p, dops + first[i], n_ops);
+        if (h->prec == 8)
+            launch_t<double>(h, amps, passes[i], threads, dops + first[i], n_ops, gpw);
+        else
+            launch_t<float>(h, amps, passes[i], threads, dops + first[i], n_ops, gpw);
     }
 }
 

@@ -76,13 +76,14 @@ def test_dense_mode_validation():
 
 
 def test_autotune_sets_valid_widths():
-    keys = ("logt_f64", "low_f64", "logt_f32", "low_f32", "lds_max_target_f32", "dense_f")
+    keys = ("logt_f64", "low_f64", "logt_f32", "low_f32", "lds_max_target_f32", "dense_f",
+            "rthreads_f64", "rthreads_f32")
     saved = {k: L.tuning_get(k) for k in keys}
     saved_reg = L.tuning_get("reg_tile")
     try:
         rep = L.autotune(4, 22)
-        assert rep["tile_best"] == [L.tuning_get("reg_tile"), L.tuning_get("logt_f32"),
-                                    L.tuning_get("low_f32")]
+        kern = L.tuning_get("rthreads_f32") if L.tuning_get("reg_tile") else 0
+        assert rep["tile_best"] == [kern, L.tuning_get("logt_f32"), L.tuning_get("low_f32")]
         assert L.tuning_get("dense_f") in (4, 5)
         # the tuned configuration still computes the same circuit
         n = 16

@@ -35,14 +35,23 @@ def tuning():
     saved = {s: (L.tuning_get("logt_" + s), L.tuning_get("low_" + s)) for s in ("f64", "f32")}
     saved_reg = L.tuning_get("reg_tile")
 
-    def setter(reg, prec, logt, low):
+    saved_thr = {s: L.tuning_get("rthreads_" + s) for s in ("f64", "f32")}
+
+    def setter(kern, prec, logt, low):
+        # kern: 0 = LDS-resident tile kernel, 256 / 512 = register tiles
+        sfx = "f64" if prec == 8 else "f32"
         L.tuning_set("reg_tile", 1)  # admits every width the LDS kernel admits
-        _set_widths("f64" if prec == 8 else "f32", logt, low)
-        L.tuning_set("reg_tile", reg)
+        if kern:
+            _set_widths(sfx, 11, 3)
+            L.tuning_set("rthreads_" + sfx, kern)
+        _set_widths(sfx, logt, low)
+        L.tuning_set("reg_tile", 1 if kern else 0)
 
     yield setter
     L.tuning_set("reg_tile", 1)
     for s, (logt, low) in saved.items():
+        _set_widths(s, 11, 3)
+        L.tuning_set("rthreads_" + s, saved_thr[s])
         _set_widths(s, logt, low)
     L.tuning_set("reg_tile", saved_reg)
 
@@ -60,17 +69,19 @@ def _ops(rng, n, count, dtype):
     return ops
 
 
-CONFIGS = [(1, 10, 2), (1, 10, 3), (1, 11, 3), (1, 11, 5), (1, 12, 4), (1, 12, 6),
-           (0, 10, 3), (0, 11, 3)]
+CONFIGS = [(256, 10, 2), (256, 10, 3), (256, 11, 3), (256, 11, 5), (256, 12, 4),
+           (256, 12, 6), (512, 12, 4), (512, 13, 5), (0, 10, 3), (0, 11, 3)]
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [12, 15])
-@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "reg%d_t%d_low%d" % c)
+@pytest.mark.parametrize("n", [13, 15])
+@pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "k%d_t%d_low%d" % c)
 def test_tile_kernels_match_oracle(tuning, dtype, n, cfg):
-    reg, logt, low = cfg
+    kern, logt, low = cfg
     prec = 8 if dtype == np.complex128 else 4
-    tuning(reg, prec, logt, low)
+    if kern == 512 and logt == 13 and prec == 8:
+        pytest.skip("2^13 FP64 amplitudes exceed the 64 KiB LDS remap buffer")
+    tuning(kern, prec, logt, low)
     rng = np.random.default_rng(1000 * logt + 10 * low + n + prec)
     st0 = random_state(rng, n, dtype)
     ops = _ops(rng, n, 300, dtype)
@@ -91,7 +102,7 @@ def test_register_remaps_on_every_qubit(tuning, dtype):
     # register bits and group bits
     n = 18
     prec = 8 if dtype == np.complex128 else 4
-    tuning(1, prec, 12, 4)
+    tuning(256, prec, 12, 4)
     rng = np.random.default_rng(18)
     seq = []
     for t in list(range(n)) + list(range(n - 1, -1, -1)):
@@ -113,7 +124,60 @@ def test_register_remaps_on_every_qubit(tuning, dtype):
     assert np.array_equal(q.getState().view(np.uint8), want.view(np.uint8))
 
 
+@pytest.mark.parametrize("kern", [256, 512])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_register_remaps_at_512_threads(tuning, kern, dtype):
+    n = 17
+    prec = 8 if dtype == np.complex128 else 4
+    tuning(kern, prec, 12, 4)
+    rng = np.random.default_rng(170 + kern)
+    st0 = random_state(rng, n, dtype)
+    ops = _ops(rng, n, 400, dtype)
+    q = qsl_amd.Qubits(n, dtype=dtype)
+    q.setState(st0)
+    q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    want = st0.copy()
+    O.apply_ops(want, ops)
+    assert np.array_equal(q.getState().view(np.uint8), want.view(np.uint8))
+
+
 def test_tuning_rejects_lds_widths_without_register_tiles(tuning):
-    tuning(1, 8, 12, 4)  # 2^12 FP64 amplitudes: only the register kernel fits
+    tuning(256, 8, 12, 4)  # 2^12 FP64 amplitudes: only the register kernel fits
     with pytest.raises(qsl_amd.QslHipError):
         L.tuning_set("reg_tile", 0)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("cfg", [(256, 11, 3), (512, 12, 4), (0, 10, 3)],
+                         ids=lambda c: "k%d_t%d_low%d" % c)
+def test_reorder_mode_matches_program_order(tuning, dtype, cfg):
+    # commutation-aware reordering: the same product, so the state matches
+    # the oracle's program-order run to rounding (the stated FP64 bound,
+    # 1e-12 on every amplitude; FP32 to its own rounding)
+    kern, logt, low = cfg
+    prec = 8 if dtype == np.complex128 else 4
+    tuning(kern, prec, logt, low)
+    n = 16
+    rng = np.random.default_rng(4000 + logt + prec)
+    st0 = random_state(rng, n, dtype)
+    ops = _ops(rng, n, 400, dtype)
+    q = qsl_amd.Qubits(n, dtype=dtype)
+    q.setState(st0)
+    q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    want = st0.copy()
+    O.apply_ops(want, ops)
+    assert np.abs(q.getState() - want).max() <= (1e-12 if prec == 8 else 1e-5)
+
+
+def test_reorder_mode_at_22_qubits_from_zero_state(tuning):
+    from qsl_amd import workloads as W
+    tuning(256, 8, 11, 3)
+    n = 22
+    ops, _ = W.c2_random(n, 200, W.Stream(22))
+    a = qsl_amd.Qubits(n)
+    b = qsl_amd.Qubits(n)
+    a.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    b.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    sa, sb = a.getState(), b.getState()
+    assert np.abs(sa - sb).max() <= 1e-12
+    assert abs(np.vdot(sb, sb).real - 1.0) <= 1e-12
