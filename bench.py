@@ -4,11 +4,15 @@ One step = reset to |0..0>, the 200-gate seeded random circuit of config 2
 (40% RX/RY/RZ, 20% H, 20% CNOT, 20% CPhase, qubits uniform over 0..29),
 then 10^6 shots sampled from the final state.  `value` is gates per second
 over the whole timed region (reset + gates + sampling, inputs resident in
-HBM).  Printed as one JSON line with the roofline of the dominant kernel
-(the LDS tile pass, timed with HIP events on the library's stream) and the
+HBM).  The gates run in QSLHIP_MODE_REORDER: commutation-aware packing into
+register-tile passes (the same product; every run checks the full 2^30
+state against the bit-exact program-order result, max |diff| <= 1e-12).
+Printed as one JSON line with the roofline of the dominant kernel (the
+register tile pass, timed with HIP events on the library's stream) and the
 reference's Omp backend timed on the host cores on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fused|per-gate]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--mode reorder|fused|per-gate]
 """
 import argparse
 import ctypes as C
@@ -132,7 +136,7 @@ def cpu_baseline(ops, gpu_state_fn, budget_s=20.0):
 def read_traffic():
     """Measured HBM bytes per tile-pass launch (rocprofv3 FETCH_SIZE +
     WRITE_SIZE, scaled by a copy kernel's reading), if profiled."""
-    path = os.path.join(ROOT, "profiles", "k_tile_f64_n30_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "k_rtile_f64_n30_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -320,7 +324,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["fused", "per-gate"], default="fused")
+    ap.add_argument("--mode", choices=["reorder", "fused", "per-gate"], default="reorder")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for rocprofv3 counter passes")
@@ -337,15 +341,20 @@ def main():
     from qsl_amd import workloads as W
     from qsl_amd import _lib as L
 
-    mode = qsl_amd.MODE_FUSED if args.mode == "fused" else qsl_amd.MODE_PER_GATE
+    mode = {"reorder": qsl_amd.MODE_REORDER, "fused": qsl_amd.MODE_FUSED,
+            "per-gate": qsl_amd.MODE_PER_GATE}[args.mode]
     ops, stream = W.c2_random(N, NGATES)
     if args.profile:
-        # a short fixed run for rocprofv3 counter passes: the fused circuit
-        # once (15 tile passes), rotateX on a low and a high qubit, and one
+        # a short fixed run for rocprofv3 counter passes: the circuit once
+        # reordered and once in program order, rotateX on a low and a high
+        # qubit, and one
         # device-to-device copy of the 16 GiB state (2^30*16 B read and
         # written: the calibration for FETCH_SIZE / WRITE_SIZE)
         q = qsl_amd.Qubits(N, dtype=np.complex128)
-        print("profile: fused circuit", flush=True)
+        print("profile: reordered circuit", flush=True)
+        q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+        q.sync()
+        print("profile: program-order circuit", flush=True)
         q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
         q.sync()
         print("profile: rotateX t=5 x3, t=29 x3", flush=True)
@@ -360,8 +369,10 @@ def main():
         print("profile: done", flush=True)
         return
     shots_u = stream.take(NSHOTS)
-    masks, counts = L.plan(N, 8, ops)
-    npasses = len(masks)
+    npasses = len(L.plan_order(N, 8, ops, L.MODE_REORDER)[0])
+    npasses_fused = len(L.plan_order(N, 8, ops, L.MODE_FUSED)[0])
+    hl_mode = mode if mode != qsl_amd.MODE_PER_GATE else qsl_amd.MODE_REORDER
+    hl_passes = npasses if hl_mode == qsl_amd.MODE_REORDER else npasses_fused
 
     q = qsl_amd.Qubits(N, dtype=np.complex128)
     ev = HipEvents()
@@ -388,13 +399,23 @@ def main():
     q.reset()
     q.sync()
     ev.record(e0, s)
+    q.apply_ops(ops, mode=hl_mode)
+    ev.record(e1, s)
+    q.sync()
+    gate_ms = ev.elapsed_ms(e0, e1)
+    q.reset()
+    q.sync()
+    ev.record(e0, s)
     q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
     ev.record(e1, s)
     q.sync()
     fused_ms = ev.elapsed_ms(e0, e1)
+    q.reset()
+    q.apply_ops(ops, mode=hl_mode)
+    q.sync()
     state_bytes = (1 << N) * 16
     pass_bytes = 2 * state_bytes
-    tile_avg_ms = fused_ms / npasses
+    tile_avg_ms = gate_ms / hl_passes
     t_s0 = time.perf_counter()
     q.sampleAllOutcomes(shots_u)
     sample_ms = 1000 * (time.perf_counter() - t_s0)
@@ -421,7 +442,8 @@ def main():
     per_gate_ms = ev.elapsed_ms(e0, e2)
     alg_bytes = W.algorithmic_bytes(ops, N, 16)
 
-    # --- bit-identity of fused vs per-gate at full size ----------------------
+    # --- full-size checks: fused bit-identical to per-gate (program order);
+    # the reordered state within 1e-12 of it on every amplitude ------------
     q.reset()
     q.apply_ops(ops, mode=qsl_amd.MODE_PER_GATE)
     ref_state = q.getState()
@@ -430,7 +452,15 @@ def main():
     fused_state = q.getState()
     fused_bit_identical = bool(np.array_equal(ref_state.view(np.uint8),
                                               fused_state.view(np.uint8)))
-    del ref_state, fused_state
+    del fused_state
+    q.reset()
+    q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    reorder_state = q.getState()
+    reorder_diff = 0.0
+    for i in range(0, ref_state.size, 1 << 26):  # chunked: no 16 GiB temporaries
+        j = i + (1 << 26)
+        reorder_diff = max(reorder_diff, float(np.abs(reorder_state[i:j] - ref_state[i:j]).max()))
+    del ref_state, reorder_state
 
     achieved = pass_bytes / (tile_avg_ms * 1e-3) / 1e9
     traffic = read_traffic()
@@ -453,7 +483,8 @@ def main():
                         "uniform over 0-29), then 1e6 shots; step = reset + gates + "
                         "sampling",
             "nqubits": N, "gates": NGATES, "shots": NSHOTS, "mode": args.mode,
-            "seed": W.MASTER_SEED + 2, "tile_passes": npasses,
+            "seed": W.MASTER_SEED + 2, "tile_passes": hl_passes,
+            "tile_qubits": L.tuning_get("logt_f64"), "low_run": L.tuning_get("low_f64"),
         },
         "roofline": {
             "bound": "hbm",
@@ -462,17 +493,29 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
-            "kernel": "k_tile<double> (LDS tile pass; 2*2^30*16 B per launch)",
+            "kernel": "k_rtile<double,256,8> (register tile pass; unit = one pass, "
+                      "2*2^30*16 B per launch)" if L.tuning_get("reg_tile") else
+                      "k_tile<double> (LDS tile pass; 2*2^30*16 B per launch)",
             "avg_launch_ms": round(tile_avg_ms, 4),
+            "gates_per_launch": round(NGATES / hl_passes, 1),
+            "note": "a pass of ~33 gates is FP64-VALU bound, not HBM bound: see "
+                    "the SQ counter summary in profiles/",
         },
         "gate_phase": {
+            "mode": args.mode,
+            "gate_ms": round(gate_ms, 3),
+            "gates_per_s": round(NGATES / (gate_ms * 1e-3), 1),
+            "algorithmic_GBps": round(W.algorithmic_bytes(ops, N, 16) / (gate_ms * 1e-3) / 1e9, 1),
+            "reorder_max_abs_diff_vs_program_order": reorder_diff,
+            "reorder_within_1e-12": reorder_diff <= 1e-12,
+            "fused_passes": npasses_fused,
             "fused_ms": round(fused_ms, 3),
             "fused_gates_per_s": round(NGATES / (fused_ms * 1e-3), 1),
             "per_gate_ms": round(per_gate_ms, 3),
             "per_gate_gates_per_s": round(NGATES / (per_gate_ms * 1e-3), 1),
             "algorithmic_GB": round(alg_bytes / 1e9, 2),
             "per_gate_hbm_GBps": round(alg_bytes / (per_gate_ms * 1e-3) / 1e9, 1),
-            "fused_hbm_GBps_actual": round(npasses * pass_bytes / (fused_ms * 1e-3) / 1e9, 1),
+            "fused_hbm_GBps_actual": round(npasses_fused * pass_bytes / (fused_ms * 1e-3) / 1e9, 1),
             "fused_bit_identical_to_per_gate": fused_bit_identical,
             "norm_err": norm_err,
         },

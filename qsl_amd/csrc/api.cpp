@@ -545,6 +545,17 @@ int qslhip_gate2(qslhip_t h, int kind, unsigned q0, unsigned q1, double angle)
     });
 }
 
+// Tile width of a fused pass on an n-qubit state: the tuned width, narrowed
+// on small states so that at least 64 workgroups share the device, but not
+// below 2^10 while register tiles are on (their narrowest kernel).
+static int tile_width(const Tuning &tu, unsigned n, int prec)
+{
+    const int logt = prec == 8 ? tu.logt_f64 : tu.logt_f32;
+    const int floor = tu.reg_tile ? 10 : 6;
+    const int cap = (int)n - 6 > floor ? (int)n - 6 : floor;
+    return logt < cap ? logt : cap;
+}
+
 int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
 {
     return guarded([&] {
@@ -589,7 +600,7 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
             return;
         }
         const Tuning &tu = tuning();
-        int logt = h->prec == 8 ? tu.logt_f64 : tu.logt_f32;
+        int logt = tile_width(tu, h->n, h->prec);
         int low = h->prec == 8 ? tu.low_f64 : tu.low_f32;
         std::vector<TilePass> passes;
         std::vector<TileOp> tops;
@@ -599,7 +610,7 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
         } else {
             plan_tile_passes(h->n, h->prec, gates, logt, low, passes, tops);
         }
-        launch_tile_passes(h, passes, tops);
+        launch_tile_passes(h, passes, tops, nullptr, mode != QSLHIP_MODE_REORDER);
     });
 }
 
@@ -633,7 +644,7 @@ int qslhip_plan_mode(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t 
                                       o.angle));
         }
         const Tuning &tu = tuning();
-        int logt = prec == 8 ? tu.logt_f64 : tu.logt_f32;
+        int logt = tile_width(tu, nqubits, prec);
         int low = prec == 8 ? tu.low_f64 : tu.low_f32;
         std::vector<TilePass> passes;
         std::vector<TileOp> tops;

@@ -109,6 +109,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "rthreads_f64") *value = t.rthreads_f64;
         else if (k == "rthreads_f32") *value = t.rthreads_f32;
         else if (k == "rgroups") *value = t.rgroups;
+        else if (k == "reg_min_qubits") *value = t.reg_min_qubits;
         else throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
     });
 }
@@ -149,6 +150,10 @@ int qslhip_tuning_set(const char *key, int value)
                            tile_fits(u, u.logt_f32, u.low_f32, 8),
                        QSLHIP_EINVAL, "current tile widths need register tiles");
             t.reg_tile = value;
+        } else if (k == "reg_min_qubits") {
+            QH_REQUIRE(value >= 0 && value <= (int)kMaxQubits, QSLHIP_EINVAL,
+                       "reg_min_qubits must be in [0, 40]");
+            t.reg_min_qubits = value;
         } else if (k == "rgroups") {
             QH_REQUIRE(value >= 0 && value <= 1024, QSLHIP_EINVAL, "rgroups must be in [0, 1024]");
             t.rgroups = value;

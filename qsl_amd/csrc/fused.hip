@@ -119,7 +119,7 @@ void run(qslhip_state *h, const std::vector<TilePass> &passes, const TileOp *dop
 }  // namespace
 
 void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                        const std::vector<TileOp> &ops, void *amps)
+                        const std::vector<TileOp> &ops, void *amps, bool exact)
 {
     if (!amps) amps = h->amps;
     if (passes.empty() || ops.empty()) return;
@@ -130,10 +130,10 @@ void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
     }
     const Tuning &tu = tuning();
     const int threads = h->prec == 8 ? tu.rthreads_f64 : tu.rthreads_f32;
-    bool reg = tu.reg_tile != 0;
+    bool reg = tu.reg_tile != 0 && h->n >= (unsigned)tu.reg_min_qubits;
     for (const TilePass &p : passes) reg = reg && reg_tile_ok(p.logt, threads, h->prec);
     if (reg) {
-        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups);
+        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups, exact);
         return;
     }
     size_t bytes = ops.size() * sizeof(TileOp);

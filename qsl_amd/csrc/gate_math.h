@@ -166,6 +166,53 @@ __device__ __forceinline__ void pair_update(int body, cplx<T> &a, cplx<T> &b,
     }
 }
 
+// The same updates for QSLHIP_MODE_REORDER, where results are bound by a
+// tolerance instead of bit-identity: products folded into fused
+// multiply-adds (one rounding instead of two) and the Hadamard as a product
+// with RN(1/sqrt2) (within an ulp of the division).
+template <typename T>
+__device__ __forceinline__ T fmadd(T a, T b, T c)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_fma(a, b, c);
+    else return __builtin_fmaf(a, b, c);
+}
+
+template <typename T>
+__device__ __forceinline__ void pair_update_fast(int body, cplx<T> &a, cplx<T> &b, T c, T s)
+{
+    const T rsqrt2 = sizeof(T) == 8 ? (T)0x1.6a09e667f3bccp-1 : (T)0x1.6a09e6p-1f;
+    T a0r = a.re, a0i = a.im, a1r = b.re, a1i = b.im;
+    switch (body) {
+    case QSLHIP_HADAMARD:
+        a.re = (a0r + a1r) * rsqrt2; a.im = (a0i + a1i) * rsqrt2;
+        b.re = (a0r - a1r) * rsqrt2; b.im = (a0i - a1i) * rsqrt2;
+        break;
+    case QSLHIP_ROT_X:
+        a.re = fmadd(a0r, c, a1i * s); a.im = fmadd(a0i, c, -(a1r * s));
+        b.re = fmadd(a1r, c, a0i * s); b.im = fmadd(a1i, c, -(a0r * s));
+        break;
+    case QSLHIP_ROT_Y:
+        a.re = fmadd(a0r, c, -(a1r * s)); a.im = fmadd(a0i, c, -(a1i * s));
+        b.re = fmadd(a0r, s, a1r * c); b.im = fmadd(a0i, s, a1i * c);
+        break;
+    default:  // pauliX / pauliY: moves and negations, already exact
+        pair_update<T>(body, a, b, c, s);
+        break;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void one_update_fast(int body, cplx<T> &b, T c, T s)
+{
+    if (body == QSLHIP_PAULI_Z) {
+        b.re = -b.re; b.im = -b.im;
+    } else {
+        const T tr = b.re, ti = b.im;
+        b.re = fmadd(c, tr, -(s * ti));
+        b.im = fmadd(c, ti, s * tr);
+    }
+}
+
 // Update of a single amplitude whose acted-on bit(s) are 1: pauliZ /
 // controlZ / fswap's |11> negate (gates.cpp:66-81, 278-301, 536-570) or
 // phase / controlPhase (gates.cpp:103-118, 436-465).

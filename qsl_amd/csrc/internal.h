@@ -117,13 +117,17 @@ namespace qslhip {
 // ---- launchers (defined in the .hip files) -----------------------------------
 // amps: apply to this buffer (same size and precision) instead of h->amps
 void launch_gate(qslhip_state *h, const GateDesc &g, void *amps = nullptr);
+// exact: the reference's per-gate arithmetic (bit-identical to one kernel
+// per gate); false: tolerance-bound fused multiply-adds (QSLHIP_MODE_REORDER)
 void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                        const std::vector<TileOp> &ops, void *amps = nullptr);
+                        const std::vector<TileOp> &ops, void *amps = nullptr,
+                        bool exact = true);
 // the same passes with the amplitudes held in registers, `threads` (256 or
 // 512) per workgroup (regtile.hip); reg_tile_ok says whether a kernel
 // exists for the tile width
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                       const std::vector<TileOp> &ops, void *amps, int threads, int gpw);
+                       const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
+                       bool exact);
 bool reg_tile_ok(int logt, int threads, int prec);
 void launch_fill_basis(qslhip_state *h, uint64_t index);
 double device_prob(qslhip_state *h, unsigned t, unsigned outcome);
@@ -175,6 +179,7 @@ struct Tuning {
     int rthreads_f64 = 256;      // workgroup size of the register-tile kernel
     int rthreads_f32 = 256;
     int rgroups = 4;             // groups per register-tile workgroup (0: persistent)
+    int reg_min_qubits = 18;     // smaller states: LDS tiles (latency bound there)
 };
 Tuning &tuning();
 

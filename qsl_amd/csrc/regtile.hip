@@ -17,6 +17,10 @@
 // (thread part ^ register part) is the XOR of the two offsets, the register
 // parts come precomputed with the op, and one XOR per element remains.
 //
+// EX selects the arithmetic: the reference's exact per-gate formulas
+// (bit-identical), or for QSLHIP_MODE_REORDER the tolerance-bound forms with
+// fused multiply-adds (gate_math.h pair_update_fast).
+//
 // Loads and stores use the layout with registers on the top r positions,
 // so threads sit on the lowest tile qubits (coalesced).  A workgroup may
 // take several groups and prefetch the next while computing one (k_rtile).  The per-amplitude
@@ -100,16 +104,17 @@ __device__ __forceinline__ cplx<T> pick(bool p, const cplx<T> &a, const cplx<T> 
 
 // Pair op with the target on register bit I; kmask (uniform) selects the
 // pairs, tp (per lane, SEL only) the lanes whose control is set.
-template <typename T, int R, int I, int BODY, bool SEL>
+template <typename T, int R, int I, int BODY, bool SEL, bool EX>
 __device__ __forceinline__ void reg_pair(cplx<T> (&v)[R], uint32_t kmask, bool tp, T c, T s)
 {
-    QH_CASE_MARK(I, BODY + 64 * SEL);
+    QH_CASE_MARK(I, BODY + 64 * SEL + 128 * EX);
 #pragma unroll
     for (int k = 0; k < R; k++) {
         if (k & (1 << I)) continue;
         if (!((kmask >> k) & 1u)) continue;
         cplx<T> a = v[k], b = v[k | (1 << I)];
-        pair_update<T>(BODY, a, b, c, s);
+        if constexpr (EX) pair_update<T>(BODY, a, b, c, s);
+        else pair_update_fast<T>(BODY, a, b, c, s);
         if (SEL) {
             v[k] = pick(tp, a, v[k]);
             v[k | (1 << I)] = pick(tp, b, v[k | (1 << I)]);
@@ -118,58 +123,61 @@ __device__ __forceinline__ void reg_pair(cplx<T> (&v)[R], uint32_t kmask, bool t
             v[k | (1 << I)] = b;
         }
     }
-    QH_CASE_MARK(I, BODY + 64 * SEL);
+    QH_CASE_MARK(I, BODY + 64 * SEL + 128 * EX);
 }
 
 // Swap-like op on register bits IA (q0) and IB (q1): |q0=1,q1=0> paired
 // with |q0=0,q1=1>; fswap also negates |11>.
-template <typename T, int R, int IA, int IB, int BODY, bool FSWAP>
+template <typename T, int R, int IA, int IB, int BODY, bool FSWAP, bool EX>
 __device__ __forceinline__ void reg_swaplike(cplx<T> (&v)[R], T c, T s)
 {
-    QH_CASE_MARK(IA * 4 + IB, BODY + 64 * FSWAP);
+    QH_CASE_MARK(IA * 4 + IB, BODY + 64 * FSWAP + 128 * EX);
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const bool a = k & (1 << IA), b = k & (1 << IB);
-        if (a && !b) pair_update<T>(BODY, v[k], v[k ^ (1 << IA) ^ (1 << IB)], c, s);
+        if (a && !b) {
+            if constexpr (EX) pair_update<T>(BODY, v[k], v[k ^ (1 << IA) ^ (1 << IB)], c, s);
+            else pair_update_fast<T>(BODY, v[k], v[k ^ (1 << IA) ^ (1 << IB)], c, s);
+        }
         if (FSWAP && a && b) one_update<T>(QSLHIP_PAULI_Z, v[k], c, s);
     }
-    QH_CASE_MARK(IA * 4 + IB, BODY + 64 * FSWAP);
+    QH_CASE_MARK(IA * 4 + IB, BODY + 64 * FSWAP + 128 * EX);
 }
 
-template <typename T, int R, int BODY, bool SEL>
+template <typename T, int R, int BODY, bool SEL, bool EX>
 __device__ __forceinline__ void dispatch_pair(cplx<T> (&v)[R], int ia, uint32_t kmask, bool tp,
                                               T c, T s)
 {
     switch (ia) {
-    case 0: reg_pair<T, R, 0, BODY, SEL>(v, kmask, tp, c, s); break;
-    case 1: reg_pair<T, R, 1, BODY, SEL>(v, kmask, tp, c, s); break;
-    case 2: if constexpr (R > 4) reg_pair<T, R, 2, BODY, SEL>(v, kmask, tp, c, s); break;
-    case 3: if constexpr (R > 8) reg_pair<T, R, 3, BODY, SEL>(v, kmask, tp, c, s); break;
+    case 0: reg_pair<T, R, 0, BODY, SEL, EX>(v, kmask, tp, c, s); break;
+    case 1: reg_pair<T, R, 1, BODY, SEL, EX>(v, kmask, tp, c, s); break;
+    case 2: if constexpr (R > 4) reg_pair<T, R, 2, BODY, SEL, EX>(v, kmask, tp, c, s); break;
+    case 3: if constexpr (R > 8) reg_pair<T, R, 3, BODY, SEL, EX>(v, kmask, tp, c, s); break;
     default: break;
     }
 }
 
-template <typename T, int R, bool SEL>
+template <typename T, int R, bool SEL, bool EX>
 __device__ __forceinline__ void dispatch_body(cplx<T> (&v)[R], int body, int ia, uint32_t kmask,
                                               bool tp, T c, T s)
 {
     switch (body) {
-    case QSLHIP_PAULI_X: dispatch_pair<T, R, QSLHIP_PAULI_X, SEL>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_PAULI_Y: dispatch_pair<T, R, QSLHIP_PAULI_Y, SEL>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_HADAMARD: dispatch_pair<T, R, QSLHIP_HADAMARD, SEL>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_ROT_X: dispatch_pair<T, R, QSLHIP_ROT_X, SEL>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_ROT_Y: dispatch_pair<T, R, QSLHIP_ROT_Y, SEL>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_PAULI_X: dispatch_pair<T, R, QSLHIP_PAULI_X, SEL, EX>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_PAULI_Y: dispatch_pair<T, R, QSLHIP_PAULI_Y, SEL, EX>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_HADAMARD: dispatch_pair<T, R, QSLHIP_HADAMARD, SEL, EX>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_ROT_X: dispatch_pair<T, R, QSLHIP_ROT_X, SEL, EX>(v, ia, kmask, tp, c, s); break;
+    case QSLHIP_ROT_Y: dispatch_pair<T, R, QSLHIP_ROT_Y, SEL, EX>(v, ia, kmask, tp, c, s); break;
     default: break;
     }
 }
 
-template <typename T, int R, int BODY, bool FSWAP>
+template <typename T, int R, int BODY, bool FSWAP, bool EX>
 __device__ __forceinline__ void dispatch_swap(cplx<T> (&v)[R], int ia, int ib, T c, T s)
 {
 #define QH_SW(A, B)                                                                   \
     case (A) * 4 + (B):                                                               \
         if constexpr ((1 << (A)) < R && (1 << (B)) < R)                               \
-            reg_swaplike<T, R, A, B, BODY, FSWAP>(v, c, s);                           \
+            reg_swaplike<T, R, A, B, BODY, FSWAP, EX>(v, c, s);                       \
         break;
     switch (ia * 4 + ib) {
         QH_SW(0, 1) QH_SW(0, 2) QH_SW(0, 3) QH_SW(1, 0) QH_SW(1, 2) QH_SW(1, 3)
@@ -179,14 +187,15 @@ __device__ __forceinline__ void dispatch_swap(cplx<T> (&v)[R], int ia, int ib, T
 #undef QH_SW
 }
 
-template <typename T, int R, int BODY, bool SEL>
+template <typename T, int R, int BODY, bool SEL, bool EX>
 __device__ __forceinline__ void reg_one(cplx<T> (&v)[R], uint32_t kmask, bool tp, T c, T s)
 {
 #pragma unroll
     for (int k = 0; k < R; k++) {
         if (!((kmask >> k) & 1u)) continue;
         cplx<T> x = v[k];
-        one_update<T>(BODY, x, c, s);
+        if constexpr (EX) one_update<T>(BODY, x, c, s);
+        else one_update_fast<T>(BODY, x, c, s);
         v[k] = SEL ? pick(tp, x, v[k]) : x;
     }
 }
@@ -204,7 +213,7 @@ __device__ __forceinline__ float flip_sign(float x, bool neg)
 
 // rotateZ (and controlRotateZ): the half of every register follows its
 // target bit: gneg (group), kneg (register k) and the lane's own sl.
-template <typename T, int R, bool SEL>
+template <typename T, int R, bool SEL, bool EX>
 __device__ __forceinline__ void reg_rz(cplx<T> (&v)[R], uint32_t kmask, uint32_t kneg, bool gneg,
                                        bool tp, T c, T sl)
 {
@@ -212,7 +221,14 @@ __device__ __forceinline__ void reg_rz(cplx<T> (&v)[R], uint32_t kmask, uint32_t
     for (int k = 0; k < R; k++) {
         if (!((kmask >> k) & 1u)) continue;
         cplx<T> x = v[k];
-        rz_s<T>(x, c, flip_sign(sl, gneg != (bool)((kneg >> k) & 1u)));
+        const T ss = flip_sign(sl, gneg != (bool)((kneg >> k) & 1u));
+        if constexpr (EX) {
+            rz_s<T>(x, c, ss);
+        } else {
+            const T r = x.re, i = x.im;
+            x.re = fmadd(r, c, i * ss);
+            x.im = fmadd(i, c, -(r * ss));
+        }
         v[k] = SEL ? pick(tp, x, v[k]) : x;
     }
 }
@@ -241,7 +257,7 @@ __device__ __forceinline__ void wait_vmcnt()
 // count) and loads its next group into a second register set while it
 // computes and stores the current one, so HBM traffic overlaps the gate
 // arithmetic.
-template <typename T, int NT, int R>
+template <typename T, int NT, int R, bool EX>
 __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned n,
                                               RegTileArgs ta, const RegOp *__restrict__ ops,
                                               int nops, uint64_t ngroups)
@@ -320,40 +336,40 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
             const uint32_t kmask = o.kmask;
             switch (code) {
             case RC_PAIR:
-                if (sel) dispatch_body<T, R, true>(v, o.body, o.ia, kmask, tp, c, s);
-                else dispatch_body<T, R, false>(v, o.body, o.ia, kmask, tp, c, s);
+                if (sel) dispatch_body<T, R, true, EX>(v, o.body, o.ia, kmask, tp, c, s);
+                else dispatch_body<T, R, false, EX>(v, o.body, o.ia, kmask, tp, c, s);
                 break;
             case RC_SWAP:
                 switch (o.body) {
                 case QSLHIP_PAULI_X:
-                    if (o.fswap) dispatch_swap<T, R, QSLHIP_PAULI_X, true>(v, o.ia, o.ib, c, s);
-                    else dispatch_swap<T, R, QSLHIP_PAULI_X, false>(v, o.ia, o.ib, c, s);
+                    if (o.fswap) dispatch_swap<T, R, QSLHIP_PAULI_X, true, EX>(v, o.ia, o.ib, c, s);
+                    else dispatch_swap<T, R, QSLHIP_PAULI_X, false, EX>(v, o.ia, o.ib, c, s);
                     break;
                 case QSLHIP_ROT_X:
-                    dispatch_swap<T, R, QSLHIP_ROT_X, false>(v, o.ia, o.ib, c, s);
+                    dispatch_swap<T, R, QSLHIP_ROT_X, false, EX>(v, o.ia, o.ib, c, s);
                     break;
                 case QSLHIP_ROT_Y:
-                    dispatch_swap<T, R, QSLHIP_ROT_Y, false>(v, o.ia, o.ib, c, s);
+                    dispatch_swap<T, R, QSLHIP_ROT_Y, false, EX>(v, o.ia, o.ib, c, s);
                     break;
                 default:
-                    dispatch_swap<T, R, QSLHIP_HADAMARD, false>(v, o.ia, o.ib, c, s);
+                    dispatch_swap<T, R, QSLHIP_HADAMARD, false, EX>(v, o.ia, o.ib, c, s);
                     break;
                 }
                 break;
             case RC_ONE:
                 if (o.body == QSLHIP_PAULI_Z) {
-                    if (sel) reg_one<T, R, QSLHIP_PAULI_Z, true>(v, kmask, tp, c, s);
-                    else reg_one<T, R, QSLHIP_PAULI_Z, false>(v, kmask, tp, c, s);
+                    if (sel) reg_one<T, R, QSLHIP_PAULI_Z, true, EX>(v, kmask, tp, c, s);
+                    else reg_one<T, R, QSLHIP_PAULI_Z, false, EX>(v, kmask, tp, c, s);
                 } else {
-                    if (sel) reg_one<T, R, QSLHIP_PHASE, true>(v, kmask, tp, c, s);
-                    else reg_one<T, R, QSLHIP_PHASE, false>(v, kmask, tp, c, s);
+                    if (sel) reg_one<T, R, QSLHIP_PHASE, true, EX>(v, kmask, tp, c, s);
+                    else reg_one<T, R, QSLHIP_PHASE, false, EX>(v, kmask, tp, c, s);
                 }
                 break;
             default: {  // RC_RZ
                 const bool gneg = (gbase & o.gneg) != 0;
                 const T sl = flip_sign(s, (jt & o.tneg) != 0);
-                if (sel) reg_rz<T, R, true>(v, kmask, o.kneg, gneg, tp, c, sl);
-                else reg_rz<T, R, false>(v, kmask, o.kneg, gneg, tp, c, sl);
+                if (sel) reg_rz<T, R, true, EX>(v, kmask, o.kneg, gneg, tp, c, sl);
+                else reg_rz<T, R, false, EX>(v, kmask, o.kneg, gneg, tp, c, sl);
                 break;
             }
             }
@@ -506,7 +522,7 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
 
 namespace {
 
-template <typename T, int NT, int R>
+template <typename T, int NT, int R, bool EX>
 void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dops, uint32_t nops,
                 int gpw)
 {
@@ -533,12 +549,14 @@ void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dop
     // device holds at once)
     uint64_t grid;
     if (gpw > 0) {
+        // several groups per workgroup only when groups are plentiful
+        if (groups < (uint64_t)gpw * 8192) gpw = 1;
         grid = (groups + (uint64_t)gpw - 1) / (uint64_t)gpw;
     } else {
         static int resident = 0;  // per instantiation
         if (!resident) {
             int per_cu = 0, dev = 0, cus = 0;
-            QH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rtile<T, NT, R>,
+            QH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rtile<T, NT, R, EX>,
                                                                   NT, shmem));
             QH_CHECK(hipGetDevice(&dev));
             QH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -546,21 +564,21 @@ void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dop
         }
         grid = groups < (uint64_t)resident ? groups : (uint64_t)resident;
     }
-    hipLaunchKernelGGL((k_rtile<T, NT, R>), dim3((unsigned)grid), dim3(NT), shmem, h->stream,
+    hipLaunchKernelGGL((k_rtile<T, NT, R, EX>), dim3((unsigned)grid), dim3(NT), shmem, h->stream,
                        static_cast<cplx<T> *>(amps), h->n, ta, dops, (int)nops, groups);
     QH_CHECK(hipGetLastError());
 }
 
-template <typename T>
+template <typename T, bool EX>
 void launch_t(qslhip_state *h, void *amps, const TilePass &p, int nt, const RegOp *dops,
               uint32_t nops, int gpw)
 {
     const int r = p.logt - (nt == 512 ? 9 : 8);
-    if (nt == 256 && r == 2) launch_one<T, 256, 4>(h, amps, p, dops, nops, gpw);
-    else if (nt == 256 && r == 3) launch_one<T, 256, 8>(h, amps, p, dops, nops, gpw);
-    else if (nt == 256 && r == 4) launch_one<T, 256, 16>(h, amps, p, dops, nops, gpw);
-    else if (nt == 512 && r == 3) launch_one<T, 512, 8>(h, amps, p, dops, nops, gpw);
-    else if (nt == 512 && r == 4) launch_one<T, 512, 16>(h, amps, p, dops, nops, gpw);
+    if (nt == 256 && r == 2) launch_one<T, 256, 4, EX>(h, amps, p, dops, nops, gpw);
+    else if (nt == 256 && r == 3) launch_one<T, 256, 8, EX>(h, amps, p, dops, nops, gpw);
+    else if (nt == 256 && r == 4) launch_one<T, 256, 16, EX>(h, amps, p, dops, nops, gpw);
+    else if (nt == 512 && r == 3) launch_one<T, 512, 8, EX>(h, amps, p, dops, nops, gpw);
+    else if (nt == 512 && r == 4) launch_one<T, 512, 16, EX>(h, amps, p, dops, nops, gpw);
     else throw Error(QSLHIP_EINVAL, "no register-tile kernel for this tile width");
 }
 
@@ -575,7 +593,8 @@ bool reg_tile_ok(int logt, int threads, int prec)
 }
 
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                       const std::vector<TileOp> &ops, void *amps, int threads, int gpw)
+                       const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
+                       bool exact)
 {
     const int tb = threads == 512 ? 9 : 8;
     const int A = h->prec == 8 ? 16 : 8;
@@ -600,10 +619,15 @@ void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
     const RegOp *dops = static_cast<const RegOp *>(h->optab.ptr);
     for (size_t i = 0; i < passes.size(); i++) {
         const uint32_t n_ops = first[i + 1] - first[i];
-        if (h->prec == 8)
-            launch_t<double>(h, amps, passes[i], threads, dops + first[i], n_ops, gpw);
-        else
-            launch_t<float>(h, amps, passes[i], threads, dops + first[i], n_ops, gpw);
+        const TilePass &p = passes[i];
+        const RegOp *d = dops + first[i];
+        if (h->prec == 8) {
+            if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw);
+            else launch_t<double, false>(h, amps, p, threads, d, n_ops, gpw);
+        } else {
+            if (exact) launch_t<float, true>(h, amps, p, threads, d, n_ops, gpw);
+            else launch_t<float, false>(h, amps, p, threads, d, n_ops, gpw);
+        }
     }
 }
 

@@ -36,6 +36,8 @@ def tuning():
     saved_reg = L.tuning_get("reg_tile")
 
     saved_thr = {s: L.tuning_get("rthreads_" + s) for s in ("f64", "f32")}
+    saved_min = L.tuning_get("reg_min_qubits")
+    L.tuning_set("reg_min_qubits", 0)  # register tiles on small states too
 
     def setter(kern, prec, logt, low):
         # kern: 0 = LDS-resident tile kernel, 256 / 512 = register tiles
@@ -54,6 +56,7 @@ def tuning():
         L.tuning_set("rthreads_" + s, saved_thr[s])
         _set_widths(s, logt, low)
     L.tuning_set("reg_tile", saved_reg)
+    L.tuning_set("reg_min_qubits", saved_min)
 
 
 def _ops(rng, n, count, dtype):
@@ -74,7 +77,7 @@ CONFIGS = [(256, 10, 2), (256, 10, 3), (256, 11, 3), (256, 11, 5), (256, 12, 4),
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [13, 15])
+@pytest.mark.parametrize("n", [13, 19])
 @pytest.mark.parametrize("cfg", CONFIGS, ids=lambda c: "k%d_t%d_low%d" % c)
 def test_tile_kernels_match_oracle(tuning, dtype, n, cfg):
     kern, logt, low = cfg
@@ -82,6 +85,8 @@ def test_tile_kernels_match_oracle(tuning, dtype, n, cfg):
     if kern == 512 and logt == 13 and prec == 8:
         pytest.skip("2^13 FP64 amplitudes exceed the 64 KiB LDS remap buffer")
     tuning(kern, prec, logt, low)
+    # states narrower than 2^(logt+6) run narrower tiles (at least 64
+    # workgroups), never below 2^10 with register tiles on
     rng = np.random.default_rng(1000 * logt + 10 * low + n + prec)
     st0 = random_state(rng, n, dtype)
     ops = _ops(rng, n, 300, dtype)
@@ -127,7 +132,7 @@ def test_register_remaps_on_every_qubit(tuning, dtype):
 @pytest.mark.parametrize("kern", [256, 512])
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_register_remaps_at_512_threads(tuning, kern, dtype):
-    n = 17
+    n = 18
     prec = 8 if dtype == np.complex128 else 4
     tuning(kern, prec, 12, 4)
     rng = np.random.default_rng(170 + kern)
@@ -157,7 +162,7 @@ def test_reorder_mode_matches_program_order(tuning, dtype, cfg):
     kern, logt, low = cfg
     prec = 8 if dtype == np.complex128 else 4
     tuning(kern, prec, logt, low)
-    n = 16
+    n = 18
     rng = np.random.default_rng(4000 + logt + prec)
     st0 = random_state(rng, n, dtype)
     ops = _ops(rng, n, 400, dtype)
