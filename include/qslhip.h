@@ -164,6 +164,16 @@ int qslhip_plan_mode(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t 
                      int mode, uint64_t *npasses, uint64_t *tile_masks,
                      uint64_t *op_counts, uint64_t cap, uint64_t *order);
 
+/* Passes compiled at run time (no reference counterpart; host only).
+ * qslhip_jit_source writes the device source generated for pass `pass` of
+ * the register-tile plan of `ops` in `mode` (QSLHIP_MODE_FUSED or _REORDER)
+ * with the current tuning (*len = its length); qslhip_jit_check compiles a
+ * source with hipRTC for `arch` (e.g. "gfx950") and writes the compiler
+ * log.  Neither touches a device. */
+int qslhip_jit_source(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops, int mode,
+                      uint64_t pass, char *buf, size_t cap, uint64_t *len);
+int qslhip_jit_check(const char *src, const char *arch, char *log, size_t cap);
+
 /* ---- tuning (no reference counterpart) ------------------------------------
  * Keys: "logt_f64", "low_f64", "logt_f32", "low_f32" (LDS tile width and the
  * contiguous low run of fused passes), "lds_max_target_f64/_f32" (highest

@@ -65,6 +65,9 @@ SIGNATURES = {
     "qslhip_sync": (_I, [_vp]),
     "qslhip_plan": (_I, [_U, _I, _vp, _u64, C.POINTER(_u64), _vp, _vp, _u64]),
     "qslhip_plan_mode": (_I, [_U, _I, _vp, _u64, _I, C.POINTER(_u64), _vp, _vp, _u64, _vp]),
+    "qslhip_jit_source": (_I, [_U, _I, _vp, _u64, _I, _u64, C.c_char_p, C.c_size_t,
+                               C.POINTER(_u64)]),
+    "qslhip_jit_check": (_I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
     "qslhip_reset": (_I, [_vp]),
     "qslhip_set_state": (_I, [_vp, _vp, _u64]),
     "qslhip_get_state": (_I, [_vp, _vp, _u64]),
@@ -170,6 +173,25 @@ def plan_order(nqubits, prec, ops, mode=MODE_REORDER, cap=4096):
                                  ptr(masks), ptr(counts), cap, ptr(order)))
     m = min(cap, npasses.value)
     return masks[:m].copy(), counts[:m].copy(), order[:len(ops)].copy()
+
+
+def jit_source(nqubits, prec, ops, mode, pass_index):
+    """Device source the run-time compiler generates for one pass."""
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    n = _u64(0)
+    check(lib().qslhip_jit_source(nqubits, prec, ptr(ops), len(ops), mode, pass_index, None, 0,
+                                  C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(lib().qslhip_jit_source(nqubits, prec, ptr(ops), len(ops), mode, pass_index, buf,
+                                  n.value + 1, C.byref(n)))
+    return buf.value.decode()
+
+
+def jit_check(src, arch="gfx950"):
+    """Compile a generated pass with hipRTC (host only): (ok, log)."""
+    log = C.create_string_buffer(1 << 16)
+    rc = lib().qslhip_jit_check(src.encode(), arch.encode(), log, len(log))
+    return rc == OK, log.value.decode(errors="replace")
 
 
 def tuning_get(key):

@@ -10,6 +10,7 @@
 #include <random>
 
 #include "dense.h"
+#include "rtile.h"
 #include "gate_math.h"
 
 using namespace qslhip;
@@ -662,6 +663,71 @@ int qslhip_plan_mode(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t 
             if (tile_masks) tile_masks[i] = passes[i].tile_mask;
             if (op_counts) op_counts[i] = passes[i].op_count;
         }
+    });
+}
+
+int qslhip_jit_source(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops, int mode,
+                      uint64_t pass, char *buf, size_t cap, uint64_t *len)
+{
+    return guarded([&] {
+        QH_REQUIRE(len, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
+        QH_REQUIRE(prec == 4 || prec == 8, QSLHIP_EINVAL, "prec must be 4 or 8");
+        QH_REQUIRE(mode == QSLHIP_MODE_FUSED || mode == QSLHIP_MODE_REORDER, QSLHIP_EINVAL,
+                   "mode must be FUSED or REORDER");
+        QH_REQUIRE(nqubits >= 1 && nqubits <= kMaxQubits, QSLHIP_EINVAL,
+                   "nqubits must be in [1, 40]");
+        std::vector<GateDesc> gates;
+        for (uint64_t i = 0; i < nops; i++) {
+            const qslhip_op &o = ops[i];
+            bool ok1 = is_gate1(o.kind) && o.q0 < nqubits;
+            bool ok2 = is_gate2(o.kind) && o.q0 < nqubits && o.q1 < nqubits && o.q0 != o.q1;
+            QH_REQUIRE(ok1 || ok2, QSLHIP_EINVAL, "bad op " + std::to_string(i));
+            gates.push_back(make_desc(prec, o.kind, o.q0, is_gate2(o.kind) ? o.q1 : 0,
+                                      o.angle));
+        }
+        const Tuning &tu = tuning();
+        const int logt = tile_width(tu, nqubits, prec);
+        const int low = prec == 8 ? tu.low_f64 : tu.low_f32;
+        const int threads = prec == 8 ? tu.rthreads_f64 : tu.rthreads_f32;
+        std::vector<TilePass> passes;
+        std::vector<TileOp> tops;
+        std::vector<uint32_t> order;
+        if (mode == QSLHIP_MODE_REORDER)
+            plan_tile_passes_reorder(nqubits, gates, logt, low, passes, tops, order);
+        else
+            plan_tile_passes(nqubits, prec, gates, logt, low, passes, tops);
+        QH_REQUIRE(pass < passes.size(), QSLHIP_EINVAL, "no such pass");
+        const TilePass &p = passes[pass];
+        QH_REQUIRE(reg_tile_ok(p.logt, threads, prec), QSLHIP_EINVAL,
+                   "this pass runs on LDS tiles, not register tiles");
+        std::vector<RegOp> rops;
+        plan_reg_ops(tops.data() + p.op_first, p.op_count, p.logt, threads == 512 ? 9 : 8,
+                     prec == 8 ? 16 : 8, rops);
+        const std::string src = jit_pass_source(nqubits, prec, p, threads,
+                                                mode != QSLHIP_MODE_REORDER, rops.data(),
+                                                (uint32_t)rops.size());
+        *len = src.size();
+        if (buf && cap) {
+            const size_t m = src.size() < cap - 1 ? src.size() : cap - 1;
+            std::memcpy(buf, src.data(), m);
+            buf[m] = 0;
+        }
+    });
+}
+
+int qslhip_jit_check(const char *src, const char *arch, char *log, size_t cap)
+{
+    return guarded([&] {
+        QH_REQUIRE(src && arch, QSLHIP_EINVAL, "null argument");
+        std::string lg;
+        const bool ok = jit_check_source(src, arch, lg);
+        if (log && cap) {
+            const size_t m = lg.size() < cap - 1 ? lg.size() : cap - 1;
+            std::memcpy(log, lg.data(), m);
+            log[m] = 0;
+        }
+        QH_REQUIRE(ok, QSLHIP_EINVAL, "run-time compile failed: " + lg.substr(0, 400));
     });
 }
 

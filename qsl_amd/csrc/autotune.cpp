@@ -110,6 +110,9 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "rthreads_f32") *value = t.rthreads_f32;
         else if (k == "rgroups") *value = t.rgroups;
         else if (k == "reg_min_qubits") *value = t.reg_min_qubits;
+        else if (k == "jit") *value = t.jit;
+        else if (k == "jit_compiled") *value = g_jit_compiled;
+        else if (k == "jit_failed") *value = g_jit_failed;
         else throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
     });
 }
@@ -150,6 +153,9 @@ int qslhip_tuning_set(const char *key, int value)
                            tile_fits(u, u.logt_f32, u.low_f32, 8),
                        QSLHIP_EINVAL, "current tile widths need register tiles");
             t.reg_tile = value;
+        } else if (k == "jit") {
+            QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit must be 0, 1 or 2");
+            t.jit = value;
         } else if (k == "reg_min_qubits") {
             QH_REQUIRE(value >= 0 && value <= (int)kMaxQubits, QSLHIP_EINVAL,
                        "reg_min_qubits must be in [0, 40]");

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "qslhip.h"
+#include "cplx.h"
 
 namespace qslhip {
 
@@ -55,11 +56,6 @@ struct PinnedBuf {
     void release();
 };
 
-// Complex amplitude of precision T: {re, im}, same bytes as qsl::complex<T>.
-template <typename T>
-struct alignas(2 * sizeof(T)) cplx {
-    T re, im;
-};
 
 // A gate prepared for the device: kind, qubits, coefficients in the
 // handle's precision (already rounded from the host Fp computation).
@@ -180,8 +176,11 @@ struct Tuning {
     int rthreads_f32 = 256;
     int rgroups = 4;             // groups per register-tile workgroup (0: persistent)
     int reg_min_qubits = 18;     // smaller states: LDS tiles (latency bound there)
+    int jit = 1;                 // register-tile passes compiled at run time: 0 never,
+                                 // 1 once a pass structure repeats, 2 always
 };
 Tuning &tuning();
+extern int g_jit_compiled, g_jit_failed;  // passes compiled at run time / failed
 
 inline unsigned grid_for(uint64_t items, unsigned per_block)
 {

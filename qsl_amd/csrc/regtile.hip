@@ -28,229 +28,11 @@
 // to the one-gate kernels.
 #include <cstring>
 
-#include "tile_ops.h"
+#include "rtile.h"
 
 namespace qslhip {
 
-enum RegCode : int32_t { RC_REMAP = 0, RC_PAIR = 1, RC_SWAP = 2, RC_ONE = 3, RC_RZ = 4 };
-
-constexpr int kRegMaxR = 16;
-
-// One decoded op of a register-tile pass.
-struct RegOp {
-    int32_t code;    // RegCode
-    int32_t body;    // gate_math body (RC_PAIR, RC_SWAP, RC_ONE)
-    int32_t ia, ib;  // RC_PAIR: register bit of the target; RC_SWAP: of q0 and q1
-    uint32_t kmask;  // registers taking part (RC_PAIR: the bit-ia-clear one of each pair)
-    uint32_t kneg;   // RC_RZ: registers whose target bit is 1
-    uint32_t tmask;  // thread predicate (jt & tmask) == tmask
-    uint32_t tneg;   // RC_RZ: thread-bit target, as a tile-position mask
-    uint64_t gmask;  // group predicate (gbase & gmask) == gmask
-    uint64_t gneg;   // RC_RZ: target qubit outside the tile, as a mask
-    double c, s;
-    uint32_t rho;    // RC_REMAP: register positions of the new layout
-    uint32_t fswap;  // RC_SWAP: also negate |11>
-    uint32_t koff[kRegMaxR];  // RC_REMAP: swizzled LDS byte offset of register k
-};
-
-// LDS element order of the remap buffer.  A 128-byte bank sweep holds
-// B = 128 / A amplitudes (A bytes each); the slot of element e is its low
-// log2(B) bits XORed with the fold of its higher bits, so lanes whose
-// elements differ only above the slot bits still land in distinct banks.
-// The map is linear over XOR: swz(a ^ b) = swz(a) ^ swz(b).
-template <int A>
-__host__ __device__ constexpr uint32_t lds_swz(uint32_t e)
-{
-    constexpr uint32_t B = 128 / A;
-    constexpr uint32_t L = B == 8 ? 3 : 4;
-    const uint32_t x = e >> L;
-    return e ^ ((x ^ (x >> L) ^ (x >> (2 * L))) & (B - 1));
-}
-
 namespace {
-
-// Where the tile of one workgroup lives: the tile index j = t | k << tb of
-// the load/store layout maps to the state index gbase | dep(t) | kpart(k).
-struct RegTileArgs {
-    uint64_t tile_mask;          // the K tile qubits
-    uint64_t tlow;               // the tb lowest of them (thread bits)
-    uint64_t kq[4];              // qubit of tile position tb + i, as a one-bit mask
-    uint32_t koff0[kRegMaxR];    // swizzled LDS byte offsets of the load layout
-};
-
-// rotateZ's two diagonal halves without a branch on the bit: the |1> half
-// is the |0> half with s negated (x + y*(-s) is x - y*s exactly in IEEE
-// arithmetic), so the result is bit-identical to gate_math.h's rz_half.
-template <typename T>
-__device__ __forceinline__ void rz_s(cplx<T> &x, T c, T ss)
-{
-    const T r = x.re, i = x.im;
-    x.re = r * c + i * ss;
-    x.im = i * c - r * ss;
-}
-
-template <typename T>
-__device__ __forceinline__ cplx<T> pick(bool p, const cplx<T> &a, const cplx<T> &b)
-{
-    return cplx<T>{p ? a.re : b.re, p ? a.im : b.im};
-}
-
-// The register array must stay in registers: every index into it is a
-// compile-time constant.  The empty asm markers (distinct immediates per
-// instantiation) keep the optimiser from merging the cases of a runtime
-// switch over register bits into one body with a variable index, which
-// would move the array to scratch memory.
-#define QH_CASE_MARK(A, B) asm volatile("; rtile %0 %1" ::"n"(A), "n"(B))
-
-// Pair op with the target on register bit I; kmask (uniform) selects the
-// pairs, tp (per lane, SEL only) the lanes whose control is set.
-template <typename T, int R, int I, int BODY, bool SEL, bool EX>
-__device__ __forceinline__ void reg_pair(cplx<T> (&v)[R], uint32_t kmask, bool tp, T c, T s)
-{
-    QH_CASE_MARK(I, BODY + 64 * SEL + 128 * EX);
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        if (k & (1 << I)) continue;
-        if (!((kmask >> k) & 1u)) continue;
-        cplx<T> a = v[k], b = v[k | (1 << I)];
-        if constexpr (EX) pair_update<T>(BODY, a, b, c, s);
-        else pair_update_fast<T>(BODY, a, b, c, s);
-        if (SEL) {
-            v[k] = pick(tp, a, v[k]);
-            v[k | (1 << I)] = pick(tp, b, v[k | (1 << I)]);
-        } else {
-            v[k] = a;
-            v[k | (1 << I)] = b;
-        }
-    }
-    QH_CASE_MARK(I, BODY + 64 * SEL + 128 * EX);
-}
-
-// Swap-like op on register bits IA (q0) and IB (q1): |q0=1,q1=0> paired
-// with |q0=0,q1=1>; fswap also negates |11>.
-template <typename T, int R, int IA, int IB, int BODY, bool FSWAP, bool EX>
-__device__ __forceinline__ void reg_swaplike(cplx<T> (&v)[R], T c, T s)
-{
-    QH_CASE_MARK(IA * 4 + IB, BODY + 64 * FSWAP + 128 * EX);
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        const bool a = k & (1 << IA), b = k & (1 << IB);
-        if (a && !b) {
-            if constexpr (EX) pair_update<T>(BODY, v[k], v[k ^ (1 << IA) ^ (1 << IB)], c, s);
-            else pair_update_fast<T>(BODY, v[k], v[k ^ (1 << IA) ^ (1 << IB)], c, s);
-        }
-        if (FSWAP && a && b) one_update<T>(QSLHIP_PAULI_Z, v[k], c, s);
-    }
-    QH_CASE_MARK(IA * 4 + IB, BODY + 64 * FSWAP + 128 * EX);
-}
-
-template <typename T, int R, int BODY, bool SEL, bool EX>
-__device__ __forceinline__ void dispatch_pair(cplx<T> (&v)[R], int ia, uint32_t kmask, bool tp,
-                                              T c, T s)
-{
-    switch (ia) {
-    case 0: reg_pair<T, R, 0, BODY, SEL, EX>(v, kmask, tp, c, s); break;
-    case 1: reg_pair<T, R, 1, BODY, SEL, EX>(v, kmask, tp, c, s); break;
-    case 2: if constexpr (R > 4) reg_pair<T, R, 2, BODY, SEL, EX>(v, kmask, tp, c, s); break;
-    case 3: if constexpr (R > 8) reg_pair<T, R, 3, BODY, SEL, EX>(v, kmask, tp, c, s); break;
-    default: break;
-    }
-}
-
-template <typename T, int R, bool SEL, bool EX>
-__device__ __forceinline__ void dispatch_body(cplx<T> (&v)[R], int body, int ia, uint32_t kmask,
-                                              bool tp, T c, T s)
-{
-    switch (body) {
-    case QSLHIP_PAULI_X: dispatch_pair<T, R, QSLHIP_PAULI_X, SEL, EX>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_PAULI_Y: dispatch_pair<T, R, QSLHIP_PAULI_Y, SEL, EX>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_HADAMARD: dispatch_pair<T, R, QSLHIP_HADAMARD, SEL, EX>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_ROT_X: dispatch_pair<T, R, QSLHIP_ROT_X, SEL, EX>(v, ia, kmask, tp, c, s); break;
-    case QSLHIP_ROT_Y: dispatch_pair<T, R, QSLHIP_ROT_Y, SEL, EX>(v, ia, kmask, tp, c, s); break;
-    default: break;
-    }
-}
-
-template <typename T, int R, int BODY, bool FSWAP, bool EX>
-__device__ __forceinline__ void dispatch_swap(cplx<T> (&v)[R], int ia, int ib, T c, T s)
-{
-#define QH_SW(A, B)                                                                   \
-    case (A) * 4 + (B):                                                               \
-        if constexpr ((1 << (A)) < R && (1 << (B)) < R)                               \
-            reg_swaplike<T, R, A, B, BODY, FSWAP, EX>(v, c, s);                       \
-        break;
-    switch (ia * 4 + ib) {
-        QH_SW(0, 1) QH_SW(0, 2) QH_SW(0, 3) QH_SW(1, 0) QH_SW(1, 2) QH_SW(1, 3)
-        QH_SW(2, 0) QH_SW(2, 1) QH_SW(2, 3) QH_SW(3, 0) QH_SW(3, 1) QH_SW(3, 2)
-    default: break;
-    }
-#undef QH_SW
-}
-
-template <typename T, int R, int BODY, bool SEL, bool EX>
-__device__ __forceinline__ void reg_one(cplx<T> (&v)[R], uint32_t kmask, bool tp, T c, T s)
-{
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        if (!((kmask >> k) & 1u)) continue;
-        cplx<T> x = v[k];
-        if constexpr (EX) one_update<T>(BODY, x, c, s);
-        else one_update_fast<T>(BODY, x, c, s);
-        v[k] = SEL ? pick(tp, x, v[k]) : x;
-    }
-}
-
-// x with its sign flipped when neg (an exact negation: one XOR of the sign
-// bit, with no select between two negated copies).
-__device__ __forceinline__ double flip_sign(double x, bool neg)
-{
-    return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)neg << 63));
-}
-__device__ __forceinline__ float flip_sign(float x, bool neg)
-{
-    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ ((uint32_t)neg << 31));
-}
-
-// rotateZ (and controlRotateZ): the half of every register follows its
-// target bit: gneg (group), kneg (register k) and the lane's own sl.
-template <typename T, int R, bool SEL, bool EX>
-__device__ __forceinline__ void reg_rz(cplx<T> (&v)[R], uint32_t kmask, uint32_t kneg, bool gneg,
-                                       bool tp, T c, T sl)
-{
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        if (!((kmask >> k) & 1u)) continue;
-        cplx<T> x = v[k];
-        const T ss = flip_sign(sl, gneg != (bool)((kneg >> k) & 1u));
-        if constexpr (EX) {
-            rz_s<T>(x, c, ss);
-        } else {
-            const T r = x.re, i = x.im;
-            x.re = fmadd(r, c, i * ss);
-            x.im = fmadd(i, c, -(r * ss));
-        }
-        v[k] = SEL ? pick(tp, x, v[k]) : x;
-    }
-}
-
-// Workgroup barrier for the LDS remap buffer: waits for this wave's LDS
-// traffic only, so global loads already issued for the next group stay in
-// flight across it (__syncthreads would also drain them).
-__device__ __forceinline__ void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
-// s_waitcnt vmcnt(N) (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt and
-// lgkmcnt left at "no wait"): at most N of this wave's vector memory
-// operations still outstanding.  The compiler's wait insertion sees it and
-// needs no waits of its own for anything older.
-template <int N>
-__device__ __forceinline__ void wait_vmcnt()
-{
-    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
 
 // Workgroup b takes groups b, b + gridDim.x, ... (one group when the grid
 // covers them all; persistent workgroups when it is the device's resident
@@ -307,7 +89,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
 #pragma unroll
         for (int k = 0; k < R; k++) koff[k] = ta.koff0[k];
         for (int i = 0; i < nops; i++) {
-            const RegOp &o = ops[i];
+            const RegOpHead &o = ops[i].h;
             const int code = o.code;
             if (code == RC_REMAP) {
 #pragma unroll
@@ -321,7 +103,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
                 }
                 base = lds_swz<A>(jt) * A;
 #pragma unroll
-                for (int k = 0; k < R; k++) koff[k] = o.koff[k];
+                for (int k = 0; k < R; k++) koff[k] = ops[i].koff[k];
 #pragma unroll
                 for (int k = 0; k < R; k++)
                     v[k] = *reinterpret_cast<const cplx<T> *>(smem + (base ^ koff[k]));
@@ -334,15 +116,20 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
             const bool sel = tmask != 0;
             const bool tp = (jt & tmask) == tmask;
             const uint32_t kmask = o.kmask;
+            const bool full = (o.flags & RF_FULL) != 0;
             switch (code) {
             case RC_PAIR:
-                if (sel) dispatch_body<T, R, true, EX>(v, o.body, o.ia, kmask, tp, c, s);
-                else dispatch_body<T, R, false, EX>(v, o.body, o.ia, kmask, tp, c, s);
+                // a control is a thread bit (sel), a register bit (partial
+                // kmask) or outside the tile (already tested): never two
+                if (sel) dispatch_body<T, R, true, true, EX>(v, o.body, o.ia, kmask, tp, c, s);
+                else if (full) dispatch_body<T, R, false, true, EX>(v, o.body, o.ia, kmask, tp, c, s);
+                else dispatch_body<T, R, false, false, EX>(v, o.body, o.ia, kmask, tp, c, s);
                 break;
             case RC_SWAP:
                 switch (o.body) {
                 case QSLHIP_PAULI_X:
-                    if (o.fswap) dispatch_swap<T, R, QSLHIP_PAULI_X, true, EX>(v, o.ia, o.ib, c, s);
+                    if (o.flags & RF_FSWAP)
+                        dispatch_swap<T, R, QSLHIP_PAULI_X, true, EX>(v, o.ia, o.ib, c, s);
                     else dispatch_swap<T, R, QSLHIP_PAULI_X, false, EX>(v, o.ia, o.ib, c, s);
                     break;
                 case QSLHIP_ROT_X:
@@ -358,18 +145,44 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
                 break;
             case RC_ONE:
                 if (o.body == QSLHIP_PAULI_Z) {
-                    if (sel) reg_one<T, R, QSLHIP_PAULI_Z, true, EX>(v, kmask, tp, c, s);
-                    else reg_one<T, R, QSLHIP_PAULI_Z, false, EX>(v, kmask, tp, c, s);
+                    if (sel) {
+                        if (full) reg_one<T, R, QSLHIP_PAULI_Z, true, true, EX>(v, kmask, tp, c, s);
+                        else reg_one<T, R, QSLHIP_PAULI_Z, true, false, EX>(v, kmask, tp, c, s);
+                    } else {
+                        if (full) reg_one<T, R, QSLHIP_PAULI_Z, false, true, EX>(v, kmask, tp, c, s);
+                        else reg_one<T, R, QSLHIP_PAULI_Z, false, false, EX>(v, kmask, tp, c, s);
+                    }
                 } else {
-                    if (sel) reg_one<T, R, QSLHIP_PHASE, true, EX>(v, kmask, tp, c, s);
-                    else reg_one<T, R, QSLHIP_PHASE, false, EX>(v, kmask, tp, c, s);
+                    if (sel) {
+                        if (full) reg_one<T, R, QSLHIP_PHASE, true, true, EX>(v, kmask, tp, c, s);
+                        else reg_one<T, R, QSLHIP_PHASE, true, false, EX>(v, kmask, tp, c, s);
+                    } else {
+                        if (full) reg_one<T, R, QSLHIP_PHASE, false, true, EX>(v, kmask, tp, c, s);
+                        else reg_one<T, R, QSLHIP_PHASE, false, false, EX>(v, kmask, tp, c, s);
+                    }
                 }
                 break;
             default: {  // RC_RZ
-                const bool gneg = (gbase & o.gneg) != 0;
-                const T sl = flip_sign(s, (jt & o.tneg) != 0);
-                if (sel) reg_rz<T, R, true, EX>(v, kmask, o.kneg, gneg, tp, c, sl);
-                else reg_rz<T, R, false, EX>(v, kmask, o.kneg, gneg, tp, c, sl);
+                const bool gneg = (gbase & ops[i].gneg) != 0;
+                const uint32_t kneg = o.kneg;
+                if (o.tneg) {  // target on a thread bit: one sign per lane
+                    const T sl = flip_sign(s, (jt & o.tneg) != 0);
+                    if (sel) {
+                        if (full) reg_rz<T, R, true, true, true, EX>(v, kmask, kneg, gneg, tp, c, sl);
+                        else reg_rz<T, R, true, false, true, EX>(v, kmask, kneg, gneg, tp, c, sl);
+                    } else {
+                        if (full) reg_rz<T, R, false, true, true, EX>(v, kmask, kneg, gneg, tp, c, sl);
+                        else reg_rz<T, R, false, false, true, EX>(v, kmask, kneg, gneg, tp, c, sl);
+                    }
+                } else {  // a register or group target: one sign per register
+                    if (sel) {
+                        if (full) reg_rz<T, R, true, true, false, EX>(v, kmask, kneg, gneg, tp, c, s);
+                        else reg_rz<T, R, true, false, false, EX>(v, kmask, kneg, gneg, tp, c, s);
+                    } else {
+                        if (full) reg_rz<T, R, false, true, false, EX>(v, kmask, kneg, gneg, tp, c, s);
+                        else reg_rz<T, R, false, false, false, EX>(v, kmask, kneg, gneg, tp, c, s);
+                    }
+                }
                 break;
             }
             }
@@ -394,7 +207,9 @@ void targets_of(const TileOp &t, int out[2], int &cnt)
     }
 }
 
-uint32_t dep_host(uint32_t v, uint32_t mask)
+}  // namespace
+
+static uint32_t dep_host(uint32_t v, uint32_t mask)
 {
     uint32_t out = 0;
     for (int b = 0; mask; b++, mask &= mask - 1)
@@ -407,7 +222,6 @@ uint32_t swz_bytes(uint32_t e, int A)
     return A == 16 ? lds_swz<16>(e) * 16 : lds_swz<8>(e) * 8;
 }
 
-}  // namespace
 
 // Host planner: decode the TileOps of one pass (tile width K, 2^tb
 // threads, A bytes per amplitude) into register ops.  The pass starts and
@@ -430,23 +244,29 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
             if ((k >> sl) & 1) m |= 1u << k;
         return m;
     };
+    auto kset_pos_slot = [&](int sl) {  // registers whose register bit sl is 1
+        uint32_t m = 0;
+        for (int k = 0; k < R; k++)
+            if ((k >> sl) & 1) m |= 1u << k;
+        return m;
+    };
     auto remap = [&](uint32_t rho) {
         RegOp m{};
-        m.code = RC_REMAP;
-        m.rho = rho;
+        m.h.code = RC_REMAP;
+        m.h.rho = rho;
         for (int k = 0; k < R; k++) m.koff[k] = swz_bytes(dep_host((uint32_t)k, rho), A);
         out.push_back(m);
         cur = rho;
     };
-    auto pred = [&](RegOp &o, int pos, uint32_t q) {  // require qubit q (tile pos) == 1
-        if (pos < 0) o.gmask |= 1ull << q;
-        else if ((cur >> pos) & 1u) o.kmask &= kset(pos);
-        else o.tmask |= 1u << pos;
+    auto pred = [&](RegOp &op, int pos, uint32_t q) {  // require qubit q (tile pos) == 1
+        if (pos < 0) op.h.gmask |= 1ull << q;
+        else if ((cur >> pos) & 1u) op.h.kmask &= kset(pos);
+        else op.h.tmask |= 1u << pos;
     };
-    auto neg = [&](RegOp &o, int pos, uint32_t q) {  // rotateZ half follows qubit q
-        if (pos < 0) o.gneg = 1ull << q;
-        else if ((cur >> pos) & 1u) o.kneg = kset(pos);
-        else o.tneg = 1u << pos;
+    auto neg = [&](RegOp &op, int pos, uint32_t q) {  // rotateZ half follows qubit q
+        if (pos < 0) op.gneg = 1ull << q;
+        else if ((cur >> pos) & 1u) op.h.kneg = kset(pos);
+        else op.h.tneg = 1u << pos;
     };
     for (uint32_t i = 0; i < count; i++) {
         const TileOp &t = ops[i];
@@ -466,7 +286,8 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
             for (int p = K - 1; p >= 0 && __builtin_popcount(rho) < r; p--) rho |= 1u << p;
             remap(rho);
         }
-        RegOp o{};
+        RegOp op{};
+        RegOpHead &o = op.h;
         o.c = t.c;
         o.s = t.s;
         o.kmask = all_k;
@@ -481,12 +302,12 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
             break;
         case QSLHIP_ROT_Z:
             o.code = RC_RZ;
-            neg(o, t.pa, t.q0);
+            neg(op, t.pa, t.q0);
             break;
         case QSLHIP_PAULI_Z: case QSLHIP_PHASE:
             o.code = RC_ONE;
             o.body = body;
-            pred(o, t.pa, t.q0);
+            pred(op, t.pa, t.q0);
             break;
         case QSLHIP_CNOT: case QSLHIP_CY: case QSLHIP_CROT_X: case QSLHIP_CROT_Y:
         case QSLHIP_CHADAMARD:
@@ -494,30 +315,65 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
             o.body = body;
             o.ia = slot(t.pb);
             o.kmask = all_k & ~kset(t.pb);
-            pred(o, t.pa, t.q0);
+            pred(op, t.pa, t.q0);
             break;
         case QSLHIP_CROT_Z:
             o.code = RC_RZ;
-            pred(o, t.pa, t.q0);
-            neg(o, t.pb, t.q1);
+            pred(op, t.pa, t.q0);
+            neg(op, t.pb, t.q1);
             break;
         case QSLHIP_CZ: case QSLHIP_CPHASE:
             o.code = RC_ONE;
             o.body = body;
-            pred(o, t.pa, t.q0);
-            pred(o, t.pb, t.q1);
+            pred(op, t.pa, t.q0);
+            pred(op, t.pb, t.q1);
             break;
         default:  // swap-like
             o.code = RC_SWAP;
             o.body = body;
             o.ia = slot(t.pa);
             o.ib = slot(t.pb);
-            o.fswap = t.kind == QSLHIP_FSWAP;
+            if (t.kind == QSLHIP_FSWAP) o.flags |= RF_FSWAP;
             break;
         }
-        out.push_back(o);
+        const bool full = o.code == RC_PAIR ? o.kmask == (all_k & ~kset_pos_slot(o.ia))
+                                            : o.kmask == all_k;
+        if (full) o.flags |= RF_FULL;
+        // the kernel's pair dispatch relies on it: one control qubit is a
+        // thread bit, a register bit or outside the tile, never two of these
+        QH_REQUIRE(!(o.code == RC_PAIR && o.tmask && !full), QSLHIP_EINVAL,
+                   "pair op with both a thread and a register predicate");
+        out.push_back(op);
     }
     if (cur != rho0) remap(rho0);
+}
+
+RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
+{
+    const int K = p.logt, R = 1 << (K - tb);
+    RegTileArgs ta{};
+    ta.tile_mask = p.tile_mask;
+    int pos = 0;
+    for (unsigned q = 0; q < 64; q++) {
+        if (!((p.tile_mask >> q) & 1ull)) continue;
+        if (pos < tb) ta.tlow |= 1ull << q;
+        else ta.kq[pos - tb] = 1ull << q;
+        pos++;
+    }
+    QH_REQUIRE(pos == K, QSLHIP_EINVAL, "tile mask does not hold the tile width");
+    QH_REQUIRE(K - tb >= 2 && K - tb <= 4, QSLHIP_EINVAL, "2..4 register bits per thread");
+    for (int k = 0; k < R; k++) ta.koff0[k] = swz_bytes((uint32_t)k << tb, A);
+    QH_REQUIRE(((size_t)A << K) <= 64 * 1024, QSLHIP_EINVAL,
+               "register tile needs more than 64 KiB of LDS");
+    QH_REQUIRE(n >= (unsigned)K, QSLHIP_EINVAL, "tile wider than the state");
+    return ta;
+}
+
+uint64_t reg_grid(uint64_t groups, int gpw)
+{
+    // several groups per workgroup only when groups are plentiful
+    if (gpw < 1 || groups < (uint64_t)gpw * 8192) gpw = 1;
+    return (groups + (uint64_t)gpw - 1) / (uint64_t)gpw;
 }
 
 namespace {
@@ -530,29 +386,13 @@ void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dop
     constexpr int tb = NT == 256 ? 8 : 9;
     constexpr int r = R == 4 ? 2 : R == 8 ? 3 : 4;
     const int K = tb + r;
-    RegTileArgs ta{};
-    ta.tile_mask = p.tile_mask;
-    int pos = 0;
-    for (unsigned q = 0; q < 64; q++) {
-        if (!((p.tile_mask >> q) & 1ull)) continue;
-        if (pos < tb) ta.tlow |= 1ull << q;
-        else ta.kq[pos - tb] = 1ull << q;
-        pos++;
-    }
-    QH_REQUIRE(pos == K && p.logt == K, QSLHIP_EINVAL, "tile mask does not hold the tile width");
-    for (int k = 0; k < R; k++) ta.koff0[k] = swz_bytes((uint32_t)k << tb, A);
+    const RegTileArgs ta = make_tile_args(h->n, p, tb, A);
     const size_t shmem = (size_t)A << K;
-    QH_REQUIRE(shmem <= 64 * 1024, QSLHIP_EINVAL, "register tile needs more than 64 KiB of LDS");
-    QH_REQUIRE(h->n >= (unsigned)K, QSLHIP_EINVAL, "tile wider than the state");
     const uint64_t groups = 1ull << (h->n - K);
-    // gpw groups per workgroup (0: persistent, as many workgroups as the
-    // device holds at once)
     uint64_t grid;
     if (gpw > 0) {
-        // several groups per workgroup only when groups are plentiful
-        if (groups < (uint64_t)gpw * 8192) gpw = 1;
-        grid = (groups + (uint64_t)gpw - 1) / (uint64_t)gpw;
-    } else {
+        grid = reg_grid(groups, gpw);
+    } else {  // persistent: as many workgroups as the device holds at once
         static int resident = 0;  // per instantiation
         if (!resident) {
             int per_cu = 0, dev = 0, cus = 0;
@@ -608,18 +448,34 @@ void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                      rops);
     }
     first[passes.size()] = (uint32_t)rops.size();
+    // coefficient tables of the compiled passes: (c, s) per gate op
+    std::vector<double> coef;
+    std::vector<size_t> cfirst(passes.size());
+    if (tuning().jit)
+        for (size_t i = 0; i < passes.size(); i++) {
+            cfirst[i] = coef.size();
+            jit_coefficients(rops.data() + first[i], first[i + 1] - first[i], coef);
+        }
     const size_t bytes = rops.size() * sizeof(RegOp);
+    const size_t cbytes = coef.size() * sizeof(double);
     // The pinned staging buffer may still be read by the previous copy.
     if (h->staged) QH_CHECK(hipEventSynchronize(h->staged));
-    h->pin.ensure(bytes);
-    h->optab.ensure(bytes);
+    h->pin.ensure(bytes + cbytes);
+    h->optab.ensure(bytes + cbytes);
     std::memcpy(h->pin.ptr, rops.data(), bytes);
-    QH_CHECK(hipMemcpyAsync(h->optab.ptr, h->pin.ptr, bytes, hipMemcpyHostToDevice, h->stream));
+    if (cbytes) std::memcpy(static_cast<char *>(h->pin.ptr) + bytes, coef.data(), cbytes);
+    QH_CHECK(hipMemcpyAsync(h->optab.ptr, h->pin.ptr, bytes + cbytes, hipMemcpyHostToDevice,
+                            h->stream));
     QH_CHECK(hipEventRecord(h->staged, h->stream));
     const RegOp *dops = static_cast<const RegOp *>(h->optab.ptr);
+    const double *dcoef =
+        reinterpret_cast<const double *>(static_cast<char *>(h->optab.ptr) + bytes);
     for (size_t i = 0; i < passes.size(); i++) {
         const uint32_t n_ops = first[i + 1] - first[i];
         const TilePass &p = passes[i];
+        if (cbytes && jit_launch_pass(h, amps, p, threads, gpw, exact, rops.data() + first[i],
+                                      n_ops, dcoef + cfirst[i]))
+            continue;
         const RegOp *d = dops + first[i];
         if (h->prec == 8) {
             if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw);

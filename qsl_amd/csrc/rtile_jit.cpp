@@ -1,0 +1,300 @@
+// rtile_jit.cpp -- register-tile passes compiled at run time.
+//
+// The ahead-of-time kernel (regtile.hip) interprets a pass's decoded ops:
+// per op it loads the descriptor and branches to the right body, and
+// across that many-way branch the compiler keeps a second copy of the
+// register array (two moves per amplitude per op, about as much as the
+// arithmetic of a rotation).  For a pass structure that repeats -- the
+// same circuit applied again, or a variational circuit with new angles --
+// this file generates the pass as straight-line code: every op a call of
+// the same rtile_core.h helpers, with its register bits, masks, predicates
+// and remap offsets as literals.  hipRTC compiles it for the device's
+// architecture with the library's own flags (no FMA contraction, no packed
+// FP32), and the module is cached by its source text.  The angles are not
+// in the source: each op reads its (c, s) from a table, so new angles reuse
+// the compiled pass.  The arithmetic per amplitude is the interpreter's, so
+// results are the same bit for bit.
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+
+#include "rtile.h"
+
+namespace qslhip {
+namespace {
+
+#include "jit_headers.inc"  // kJitHeaders: {name, text} of the device headers
+
+struct Compiled {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    bool failed = false;
+};
+
+std::mutex g_mu;
+std::unordered_map<std::string, int> g_seen;               // pass source -> applications
+std::map<std::pair<int, std::string>, Compiled> g_cache;   // (device, source) -> module
+constexpr size_t kMaxCached = 1024;
+
+std::string hex(uint64_t x)
+{
+    char b[32];
+    std::snprintf(b, sizeof b, "0x%llxull", (unsigned long long)x);
+    return b;
+}
+
+// The pass as straight-line device code.  Mirrors k_rtile (regtile.hip):
+// same prologue, group loop with prefetch, same helper calls per op.
+std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R, bool exact,
+                       const uint32_t *koff0)
+{
+    const int r = R == 4 ? 2 : R == 8 ? 3 : 4;
+    std::ostringstream o;
+    o << "#include \"rtile_core.h\"\nusing namespace qslhip;\n"
+      << "typedef " << (prec == 8 ? "double" : "float") << " T;\n"
+      << "constexpr int R = " << R << ", NT = " << NT << ", A = " << (prec == 8 ? 16 : 8)
+      << ", r = " << r << ";\n"
+      << "constexpr bool EX = " << (exact ? "true" : "false") << ";\n"
+      << "extern \"C\" __global__ __launch_bounds__(NT) void qslhip_pass(\n"
+         "    cplx<T> *__restrict__ st, unsigned n, RegTileArgs ta,\n"
+         "    const double *__restrict__ cf, unsigned long long ngroups)\n{\n"
+         "    extern __shared__ __align__(16) unsigned char smem[];\n"
+         "    const uint32_t t = threadIdx.x;\n"
+         "    const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1);\n"
+         "    const uint64_t outside = full & ~ta.tile_mask;\n"
+         "    const uint64_t tthread = deposit((uint64_t)t, ta.tlow);\n"
+         "    uint64_t kpart[R];\n"
+         "#pragma unroll\n"
+         "    for (int k = 0; k < R; k++) {\n"
+         "        uint64_t kp = 0;\n"
+         "#pragma unroll\n"
+         "        for (int i = 0; i < r; i++)\n"
+         "            if ((k >> i) & 1) kp |= ta.kq[i];\n"
+         "        kpart[k] = kp;\n"
+         "    }\n"
+         "    uint64_t g = blockIdx.x;\n"
+         "    if (g >= ngroups) return;\n"
+         "    uint64_t gbase_n = deposit(g, outside);\n"
+         "    cplx<T> v[R], vn[R];\n"
+         "#pragma unroll\n"
+         "    for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];\n"
+         "    wait_vmcnt<0>();\n"
+         "    for (;;) {\n"
+         "        const uint64_t gbase = gbase_n;\n"
+         "#pragma unroll\n"
+         "        for (int k = 0; k < R; k++) v[k] = vn[k];\n"
+         "        const uint64_t gn = g + gridDim.x;\n"
+         "        const bool more = gn < ngroups;\n"
+         "        if (more) {\n"
+         "            gbase_n = deposit(gn, outside);\n"
+         "#pragma unroll\n"
+         "            for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];\n"
+         "            wait_vmcnt<2 * R>();\n"
+         "        } else {\n"
+         "            wait_vmcnt<R>();\n"
+         "        }\n"
+         "        uint32_t jt = t;\n"
+         "        uint32_t base = lds_swz<A>(jt) * A;\n"
+         "        (void)gbase; (void)jt; (void)base;\n";
+    std::vector<uint32_t> kcur(koff0, koff0 + R);
+    int j = 0;  // coefficient pair index
+    for (uint32_t i = 0; i < nops; i++) {
+        const RegOpHead &h = ops[i].h;
+        if (h.code == RC_REMAP) {
+            o << "        {\n";
+            for (int k = 0; k < R; k++)
+                o << "            *reinterpret_cast<cplx<T> *>(smem + (base ^ " << kcur[k]
+                  << "u)) = v[" << k << "];\n";
+            o << "            lds_barrier();\n            jt = t;\n";
+            for (uint32_t rho = h.rho; rho; rho &= rho - 1) {
+                const uint32_t lowm = (rho & (0u - rho)) - 1;
+                o << "            jt = (jt & " << lowm << "u) | ((jt & " << ~lowm
+                  << "u) << 1);\n";
+            }
+            o << "            base = lds_swz<A>(jt) * A;\n";
+            for (int k = 0; k < R; k++) {
+                kcur[k] = ops[i].koff[k];
+                o << "            v[" << k << "] = *reinterpret_cast<const cplx<T> *>(smem + (base ^ "
+                  << kcur[k] << "u));\n";
+            }
+            o << "            lds_barrier();\n        }\n";
+            continue;
+        }
+        const bool sel = h.tmask != 0;
+        o << "        {\n";
+        if (h.gmask)
+            o << "          if ((gbase & " << hex(h.gmask) << ") == " << hex(h.gmask) << ") {\n";
+        o << "            const T c = (T)cf[" << 2 * j << "], s = (T)cf[" << 2 * j + 1 << "];\n";
+        if (sel)
+            o << "            const bool tp = (jt & " << h.tmask << "u) == " << h.tmask << "u;\n";
+        else
+            o << "            const bool tp = false;\n";
+        const char *S = sel ? "true" : "false";
+        switch (h.code) {
+        case RC_PAIR:
+            o << "            reg_pair<T, R, " << h.ia << ", " << h.body << ", " << S
+              << ", false, EX>(v, " << h.kmask << "u, tp, c, s);\n";
+            break;
+        case RC_SWAP:
+            o << "            reg_swaplike<T, R, " << h.ia << ", " << h.ib << ", " << h.body << ", "
+              << ((h.flags & RF_FSWAP) ? "true" : "false") << ", EX>(v, c, s);\n";
+            break;
+        case RC_ONE:
+            o << "            reg_one<T, R, " << h.body << ", " << S << ", false, EX>(v, "
+              << h.kmask << "u, tp, c, s);\n";
+            break;
+        default: {  // RC_RZ
+            const bool lane = h.tneg != 0;
+            if (ops[i].gneg)
+                o << "            const bool gneg = (gbase & " << hex(ops[i].gneg) << ") != 0;\n";
+            else
+                o << "            const bool gneg = false;\n";
+            if (lane)
+                o << "            const T sl = flip_sign(s, (jt & " << h.tneg << "u) != 0);\n";
+            else
+                o << "            const T sl = s;\n";
+            o << "            reg_rz<T, R, " << S << ", false, " << (lane ? "true" : "false")
+              << ", EX>(v, " << h.kmask << "u, " << h.kneg << "u, gneg, tp, c, sl);\n";
+            break;
+        }
+        }
+        o << "            (void)tp;\n";
+        if (h.gmask) o << "          }\n";
+        o << "        }\n";
+        j++;
+    }
+    o << "#pragma unroll\n"
+         "        for (int k = 0; k < R; k++) st[gbase | tthread | kpart[k]] = v[k];\n"
+         "        if (!more) break;\n"
+         "        g = gn;\n"
+         "    }\n}\n";
+    return o.str();
+}
+
+// hipRTC compile of a pass for `arch`; the code object in `code`, the log in
+// `log`.
+bool rtc_compile(const std::string &src, const std::string &arch_name, std::vector<char> &code,
+                 std::string &log)
+{
+    const int nh = (int)(sizeof(kJitHeaders) / sizeof(kJitHeaders[0]));
+    std::vector<const char *> names, texts;
+    for (int i = 0; i < nh; i++) {
+        names.push_back(kJitHeaders[i][0]);
+        texts.push_back(kJitHeaders[i][1]);
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "qslhip_pass.hip", nh, texts.data(),
+                            names.data()) != HIPRTC_SUCCESS)
+        return false;
+    const std::string arch = std::string("--offload-arch=") + arch_name;
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++20", "-ffp-contract=off",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    size_t ls = 0;
+    if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+        log.assign(ls, '\0');
+        hiprtcGetProgramLog(prog, &log[0]);
+    }
+    bool ok = rc == HIPRTC_SUCCESS;
+    if (ok) {
+        size_t cs = 0;
+        ok = hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0;
+        if (ok) {
+            code.resize(cs);
+            ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+        }
+    }
+    hiprtcDestroyProgram(&prog);
+    return ok;
+}
+
+bool compile(int dev, const std::string &src, Compiled &out)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+    std::vector<char> code;
+    std::string log;
+    if (!rtc_compile(src, prop.gcnArchName, code, log)) return false;
+    if (hipModuleLoadData(&out.mod, code.data()) != hipSuccess) return false;
+    if (hipModuleGetFunction(&out.fn, out.mod, "qslhip_pass") != hipSuccess) return false;
+    return true;
+}
+
+}  // namespace
+
+std::string jit_pass_source(unsigned n, int prec, const TilePass &p, int threads, bool exact,
+                            const RegOp *ops, uint32_t nops)
+{
+    const int tb = threads == 512 ? 9 : 8;
+    const RegTileArgs ta = make_tile_args(n, p, tb, prec == 8 ? 16 : 8);
+    return gen_source(ops, nops, prec, threads, 1 << (p.logt - tb), exact, ta.koff0);
+}
+
+bool jit_check_source(const std::string &src, const std::string &arch, std::string &log)
+{
+    std::vector<char> code;
+    return rtc_compile(src, arch, code, log);
+}
+
+int g_jit_compiled = 0, g_jit_failed = 0;  // counters (tuning keys jit_compiled / jit_failed)
+
+void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef)
+{
+    for (uint32_t i = 0; i < nops; i++) {
+        if (ops[i].h.code == RC_REMAP) continue;
+        coef.push_back(ops[i].h.c);
+        coef.push_back(ops[i].h.s);
+    }
+}
+
+bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads, int gpw,
+                     bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef)
+{
+    const int mode = tuning().jit;
+    if (mode == 0) return false;
+    const int tb = threads == 512 ? 9 : 8;
+    const int A = h->prec == 8 ? 16 : 8;
+    const RegTileArgs ta = make_tile_args(h->n, p, tb, A);
+    const int R = 1 << (p.logt - tb);
+    const std::string src = gen_source(ops, nops, h->prec, threads, R, exact, ta.koff0);
+    hipFunction_t fn = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        const int seen = ++g_seen[src];
+        if (mode == 1 && seen < 2) return false;  // compile a structure once it repeats
+        auto key = std::make_pair(h->device, src);
+        auto it = g_cache.find(key);
+        if (it == g_cache.end()) {
+            if (g_cache.size() >= kMaxCached) return false;
+            Compiled c;
+            if (!compile(h->device, src, c)) {
+                c.failed = true;
+                g_jit_failed++;
+            } else {
+                g_jit_compiled++;
+            }
+            it = g_cache.emplace(key, c).first;
+        }
+        if (it->second.failed) return false;
+        fn = it->second.fn;
+    }
+    const uint64_t groups = 1ull << (h->n - p.logt);
+    const unsigned grid = (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1);
+    const size_t shmem = (size_t)A << p.logt;
+    void *st = amps;
+    unsigned n = h->n;
+    RegTileArgs targ = ta;
+    const double *cf = dev_coef;
+    unsigned long long ng = groups;
+    void *args[] = {&st, &n, &targ, &cf, &ng};
+    QH_CHECK(hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)threads, 1, 1, (unsigned)shmem,
+                                   h->stream, args, nullptr));
+    return true;
+}
+
+}  // namespace qslhip

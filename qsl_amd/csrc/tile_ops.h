@@ -15,19 +15,6 @@
 
 namespace qslhip {
 
-// Deposit the low bits of v into the set bits of mask (ascending).
-__device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask)
-{
-    uint64_t r = 0;
-    while (mask) {
-        uint64_t low = mask & (0 - mask);
-        if (v & 1) r |= low;
-        v >>= 1;
-        mask ^= low;
-    }
-    return r;
-}
-
 // Identity element order of an LDS tile (see mfma.hip for a swizzled one).
 struct NoSwizzle {
     __device__ __forceinline__ uint32_t operator()(uint32_t e) const { return e; }

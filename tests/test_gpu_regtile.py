@@ -186,3 +186,75 @@ def test_reorder_mode_at_22_qubits_from_zero_state(tuning):
     sa, sb = a.getState(), b.getState()
     assert np.abs(sa - sb).max() <= 1e-12
     assert abs(np.vdot(sb, sb).real - 1.0) <= 1e-12
+
+
+@pytest.fixture
+def jit_mode():
+    saved = L.tuning_get("jit")
+
+    def setter(v):
+        L.tuning_set("jit", v)
+
+    yield setter
+    L.tuning_set("jit", saved)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("cfg", [(256, 10, 3), (256, 11, 3), (512, 12, 4)],
+                         ids=lambda c: "k%d_t%d_low%d" % c)
+def test_compiled_passes_match_oracle(tuning, jit_mode, dtype, cfg):
+    # every pass compiled at run time (jit = 2): program order bit for bit,
+    # the reordered mode to rounding
+    kern, logt, low = cfg
+    prec = 8 if dtype == np.complex128 else 4
+    tuning(kern, prec, logt, low)
+    jit_mode(2)
+    before = L.tuning_get("jit_compiled")
+    n = 18
+    rng = np.random.default_rng(7000 + logt + prec)
+    st0 = random_state(rng, n, dtype)
+    ops = _ops(rng, n, 150, dtype)
+    want = st0.copy()
+    O.apply_ops(want, ops)
+    q = qsl_amd.Qubits(n, dtype=dtype)
+    q.setState(st0)
+    q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    got = q.getState()
+    assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
+    q.setState(st0)
+    q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    assert np.abs(q.getState() - want).max() <= (1e-12 if prec == 8 else 1e-5)
+    assert L.tuning_get("jit_compiled") > before
+    assert L.tuning_get("jit_failed") == 0
+
+
+@pytest.mark.parametrize("mode", [qsl_amd.MODE_FUSED, qsl_amd.MODE_REORDER])
+def test_compiled_pass_equals_interpreted_pass(tuning, jit_mode, mode):
+    # jit = 1: the first application interprets the decoded ops, the second
+    # runs the compiled pass; the same arithmetic in the same order gives the
+    # same bits, also with new angles (the compiled pass is angle-free)
+    tuning(256, 8, 11, 3)
+    jit_mode(1)
+    n = 19
+    rng = np.random.default_rng(900 + mode)
+    st0 = random_state(rng, n, np.complex128)
+    ops = _ops(rng, n, 120, np.complex128)
+    q = qsl_amd.Qubits(n)
+    q.setState(st0)
+    q.apply_ops(ops, mode=mode)
+    first = q.getState()
+    before = L.tuning_get("jit_compiled")
+    q.setState(st0)
+    q.apply_ops(ops, mode=mode)
+    assert np.array_equal(q.getState().view(np.uint8), first.view(np.uint8))
+    assert L.tuning_get("jit_compiled") > before
+    ops2 = ops.copy()
+    ops2["angle"] = rng.uniform(-6, 6, len(ops))
+    q.setState(st0)
+    q.apply_ops(ops2, mode=mode)  # compiled passes, new coefficients
+    want = st0.copy()
+    O.apply_ops(want, ops2)
+    if mode == qsl_amd.MODE_FUSED:
+        assert np.array_equal(q.getState().view(np.uint8), want.view(np.uint8))
+    else:
+        assert np.abs(q.getState() - want).max() <= 1e-12
