@@ -136,7 +136,7 @@ def cpu_baseline(ops, gpu_state_fn, budget_s=20.0):
 def read_traffic():
     """Measured HBM bytes per tile-pass launch (rocprofv3 FETCH_SIZE +
     WRITE_SIZE, scaled by a copy kernel's reading), if profiled."""
-    path = os.path.join(ROOT, "profiles", "k_rtile_f64_n30_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "qslhip_pass_f64_n30_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -345,18 +345,21 @@ def main():
             "per-gate": qsl_amd.MODE_PER_GATE}[args.mode]
     ops, stream = W.c2_random(N, NGATES)
     if args.profile:
-        # a short fixed run for rocprofv3 counter passes: the circuit once
-        # reordered and once in program order, rotateX on a low and a high
+        # a short fixed run for rocprofv3 counter passes: the circuit twice
+        # reordered and twice in program order (the second application of a
+        # pass structure runs its compiled pass), rotateX on a low and a high
         # qubit, and one
         # device-to-device copy of the 16 GiB state (2^30*16 B read and
         # written: the calibration for FETCH_SIZE / WRITE_SIZE)
         q = qsl_amd.Qubits(N, dtype=np.complex128)
-        print("profile: reordered circuit", flush=True)
-        q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
-        q.sync()
-        print("profile: program-order circuit", flush=True)
-        q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
-        q.sync()
+        for rep in (1, 2):  # the second application runs the compiled passes
+            print(f"profile: reordered circuit ({rep})", flush=True)
+            q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+            q.sync()
+        for rep in (1, 2):
+            print(f"profile: program-order circuit ({rep})", flush=True)
+            q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+            q.sync()
         print("profile: rotateX t=5 x3, t=29 x3", flush=True)
         for t in (5, 29):
             for _ in range(3):
@@ -383,8 +386,14 @@ def main():
         q.apply_ops(ops, mode=mode)
         return q.sampleAllOutcomes(shots_u)
 
+    warm_ms = []
+    jit0 = L.tuning_get("jit_compiled")
     for _ in range(args.warmup):
+        tw = time.perf_counter()
         out = step()
+        q.sync()
+        warm_ms.append(round(1000 * (time.perf_counter() - tw), 1))
+    jit_passes = L.tuning_get("jit_compiled") - jit0
     q.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -403,13 +412,18 @@ def main():
     ev.record(e1, s)
     q.sync()
     gate_ms = ev.elapsed_ms(e0, e1)
-    q.reset()
-    q.sync()
-    ev.record(e0, s)
-    q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
-    ev.record(e1, s)
-    q.sync()
-    fused_ms = ev.elapsed_ms(e0, e1)
+    # program order: the first application interprets the decoded ops, the
+    # second compiles the passes (untimed), the third runs them compiled
+    fused_runs = []
+    for _ in range(3):
+        q.reset()
+        q.sync()
+        ev.record(e0, s)
+        q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+        ev.record(e1, s)
+        q.sync()
+        fused_runs.append(ev.elapsed_ms(e0, e1))
+    fused_interp_ms, fused_ms = fused_runs[0], fused_runs[2]
     q.reset()
     q.apply_ops(ops, mode=hl_mode)
     q.sync()
@@ -493,13 +507,20 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
-            "kernel": "k_rtile<double,256,8> (register tile pass; unit = one pass, "
-                      "2*2^30*16 B per launch)" if L.tuning_get("reg_tile") else
-                      "k_tile<double> (LDS tile pass; 2*2^30*16 B per launch)",
+            "kernel": ("qslhip_pass (register tile pass compiled at run time; "
+                       if jit_passes else "k_rtile<double,256,8> (register tile pass; ")
+                      + "unit = one pass, 2*2^30*16 B per launch)",
             "avg_launch_ms": round(tile_avg_ms, 4),
             "gates_per_launch": round(NGATES / hl_passes, 1),
-            "note": "a pass of ~33 gates is FP64-VALU bound, not HBM bound: see "
+            "note": "HBM bytes per pass are the algorithmic bytes (traffic); the gap "
+                    "to the peak is the FP64 gate arithmetic of ~33 gates per pass: see "
                     "the SQ counter summary in profiles/",
+        },
+        "jit": {
+            "mode": L.tuning_get("jit"),
+            "passes_compiled_in_warmup": jit_passes,
+            "warmup_step_ms": warm_ms,
+            "failed": L.tuning_get("jit_failed"),
         },
         "gate_phase": {
             "mode": args.mode,
@@ -511,6 +532,7 @@ def main():
             "fused_passes": npasses_fused,
             "fused_ms": round(fused_ms, 3),
             "fused_gates_per_s": round(NGATES / (fused_ms * 1e-3), 1),
+            "fused_interpreted_ms": round(fused_interp_ms, 3),
             "per_gate_ms": round(per_gate_ms, 3),
             "per_gate_gates_per_s": round(NGATES / (per_gate_ms * 1e-3), 1),
             "algorithmic_GB": round(alg_bytes / 1e9, 2),
