@@ -104,9 +104,26 @@ qslhip_state *alloc_handle(unsigned n, int prec, bool np, unsigned k)
 
 void reset_impl(qslhip_t h)
 {
-    // Default: |0...0> (default.cpp:57-65).  NP: the lowest weight-k string
-    // 2^k - 1, whose colex rank is 0 (np.cpp:68-79).
-    launch_fill_basis(h, 0);
+    // Default: |0...0> (default.cpp:57-65), recorded and written by the next
+    // reader (materialize) or synthesized by the next gate pass.  NP: the
+    // lowest weight-k string 2^k - 1, whose colex rank is 0 (np.cpp:68-79).
+    if (h->np) launch_fill_basis(h, 0);
+    else h->pending_basis = 0;
+}
+
+// Writes a recorded basis state into the vector, before anything reads it.
+void materialize(qslhip_t h)
+{
+    if (h->pending_basis == qslhip_state::kNoBasis) return;
+    launch_fill_basis(h, h->pending_basis);
+    h->pending_basis = qslhip_state::kNoBasis;
+}
+
+// A handle whose vector holds its state.
+void require_state(qslhip_t h)
+{
+    require_handle(h);
+    materialize(h);
 }
 
 template <typename Fp>
@@ -279,7 +296,7 @@ int qslhip_destroy(qslhip_t h)
 int qslhip_clone(qslhip_t src, qslhip_t *out)
 {
     return guarded([&] {
-        require_handle(src);
+        require_state(src);
         QH_REQUIRE(out, QSLHIP_EINVAL, "null output handle");
         *out = nullptr;
         qslhip_t h = alloc_handle(src->n, src->prec, src->np, src->k);
@@ -295,8 +312,9 @@ int qslhip_assign(qslhip_t dst, qslhip_t src)
 {
     return guarded([&] {
         require_handle(dst);
-        require_handle(src);
+        require_state(src);
         if (dst == src) return;
+        dst->pending_basis = qslhip_state::kNoBasis;  // overwritten
         // default.cpp:94-105 / np.cpp:119-131
         QH_REQUIRE(dst->n == src->n, QSLHIP_ESIZE,
                    "Cannot assign Qubits object vector from different number of qubits");
@@ -374,6 +392,7 @@ int qslhip_set_state(qslhip_t h, const void *host, uint64_t count)
         QH_REQUIRE(count == (1ull << h->n), QSLHIP_ESIZE,
                    "Cannot assign state vector from different number of qubits");
         if (!h->np) {
+            h->pending_basis = qslhip_state::kNoBasis;  // overwritten
             QH_CHECK(hipMemcpyAsync(h->amps, host, count * h->amp_bytes(),
                                     hipMemcpyHostToDevice, h->stream));
             QH_CHECK(hipStreamSynchronize(h->stream));
@@ -408,7 +427,7 @@ int qslhip_set_state(qslhip_t h, const void *host, uint64_t count)
 int qslhip_get_state(qslhip_t h, void *host, uint64_t count)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(host, QSLHIP_EINVAL, "null host pointer");
         QH_REQUIRE(count == (1ull << h->n), QSLHIP_ESIZE, "count must be 2^nqubits");
         if (!h->np) {
@@ -430,7 +449,7 @@ int qslhip_get_state(qslhip_t h, void *host, uint64_t count)
 int qslhip_get_amps(qslhip_t h, void *host, uint64_t first, uint64_t count)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(host || count == 0, QSLHIP_EINVAL, "null host pointer");
         QH_REQUIRE(first <= h->namps && count <= h->namps - first, QSLHIP_EINVAL,
                    "amplitude range out of bounds");
@@ -445,7 +464,7 @@ int qslhip_get_amps(qslhip_t h, void *host, uint64_t first, uint64_t count)
 int qslhip_set_amps(qslhip_t h, const void *host, uint64_t first, uint64_t count)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(host || count == 0, QSLHIP_EINVAL, "null host pointer");
         QH_REQUIRE(first <= h->namps && count <= h->namps - first, QSLHIP_EINVAL,
                    "amplitude range out of bounds");
@@ -463,7 +482,7 @@ int qslhip_set_basis(qslhip_t h, uint64_t index)
         QH_REQUIRE(h->n >= 64 || index < (1ull << h->n), QSLHIP_EINVAL,
                    "basis index out of range");
         if (!h->np) {
-            launch_fill_basis(h, index);
+            h->pending_basis = index;  // written by the next reader or gate pass
             return;
         }
         // np.cpp:100-116: nones follows the index's popcount
@@ -525,7 +544,7 @@ static void validate_gate(qslhip_t h, int kind, unsigned q0, unsigned q1)
 int qslhip_gate1(qslhip_t h, int kind, unsigned targ, double angle)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(is_gate1(kind), QSLHIP_EINVAL, "not a one-qubit gate kind");
         validate_gate(h, kind, targ, 0);
         GateDesc g = make_desc(h->prec, kind, targ, 0, angle);
@@ -537,7 +556,7 @@ int qslhip_gate1(qslhip_t h, int kind, unsigned targ, double angle)
 int qslhip_gate2(qslhip_t h, int kind, unsigned q0, unsigned q1, double angle)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(is_gate2(kind), QSLHIP_EINVAL, "not a two-qubit gate kind");
         validate_gate(h, kind, q0, q1);
         GateDesc g = make_desc(h->prec, kind, q0, q1, angle);
@@ -580,6 +599,7 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
                                       is_gate2(o.kind) ? o.q1 : 0, o.angle));
         }
         if (mode == QSLHIP_MODE_DENSE) {
+            materialize(h);
             // block matrices from FP64 coefficients, rounded once to FP32
             std::vector<GateDesc> g64;
             g64.reserve(nops);
@@ -594,6 +614,7 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
             return;
         }
         if (mode == QSLHIP_MODE_PER_GATE || h->np) {
+            materialize(h);
             for (const GateDesc &g : gates) {
                 if (h->np) np_launch_gate(h, g);
                 else launch_gate(h, g);
@@ -611,7 +632,10 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
         } else {
             plan_tile_passes(h->n, h->prec, gates, logt, low, passes, tops);
         }
-        launch_tile_passes(h, passes, tops, nullptr, mode != QSLHIP_MODE_REORDER);
+        // a recorded basis state is synthesized by the first pass (or written first)
+        launch_tile_passes(h, passes, tops, nullptr, mode != QSLHIP_MODE_REORDER,
+                           h->pending_basis);
+        h->pending_basis = qslhip_state::kNoBasis;
     });
 }
 
@@ -793,7 +817,7 @@ int qslhip_plan_dense(unsigned nqubits, const qslhip_op *ops, uint64_t nops, int
 int qslhip_prob(qslhip_t h, unsigned targ, unsigned outcome, double *p)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(p, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
         *p = h->np ? np_prob(h, targ, outcome) : device_prob(h, targ, outcome);
@@ -803,7 +827,7 @@ int qslhip_prob(qslhip_t h, unsigned targ, unsigned outcome, double *p)
 int qslhip_collapse(qslhip_t h, unsigned targ, unsigned outcome, double factor)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
         if (h->np) np_collapse(h, targ, outcome, factor);
         else launch_collapse(h, targ, outcome, factor);
@@ -819,7 +843,7 @@ static double as_fp(qslhip_t h, double u)
 int qslhip_measure(qslhip_t h, unsigned targ, double u, int *outcome)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(outcome, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(targ < h->n, QSLHIP_EINVAL, "qubit index out of range");
         // measure.cpp:81-107, with prob0 and the factor in FP64
@@ -844,7 +868,7 @@ int qslhip_measure(qslhip_t h, unsigned targ, double u, int *outcome)
 int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(p, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
         // measure.cpp:149-161
@@ -860,7 +884,7 @@ int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p)
 int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots, uint64_t *out)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE((u && out) || nshots == 0, QSLHIP_EINVAL, "null array");
         std::vector<double> uf;
         const double *src = u;
@@ -878,7 +902,7 @@ int qslhip_sample_all2(qslhip_t h, const double *u, uint64_t nshots,
                        uint64_t *counts, uint64_t ncounts)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(counts && (u || nshots == 0), QSLHIP_EINVAL, "null array");
         QH_REQUIRE(ncounts == (1ull << h->n), QSLHIP_ESIZE, "ncounts must be 2^nqubits");
         std::vector<double> uf(u, u + nshots);
@@ -896,7 +920,7 @@ int qslhip_sample_all2(qslhip_t h, const double *u, uint64_t nshots,
 int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(outcome, QSLHIP_EINVAL, "null output");
         double uu = as_fp(h, u);
         uint64_t o = 0;
@@ -904,7 +928,7 @@ int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome)
         else device_sample(h, &uu, 1, &o);
         // measure.cpp:110-123: collapse onto the outcome
         if (h->np) launch_fill_basis(h, np_rank_host(o));
-        else launch_fill_basis(h, o);
+        else h->pending_basis = o;  // the collapsed state, written when next read
         QH_CHECK(hipStreamSynchronize(h->stream));
         *outcome = o;
     });
@@ -914,7 +938,7 @@ int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
                   uint64_t *counts)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(counts && (u || nshots == 0), QSLHIP_EINVAL, "null array");
         QH_REQUIRE(targ < h->n, QSLHIP_EINVAL, "qubit index out of range");
         // measure.cpp:191-216: one prob, then u < prob0 counts as 0
@@ -930,7 +954,7 @@ int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
 int qslhip_norm(qslhip_t h, double *norm)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(norm, QSLHIP_EINVAL, "null output");
         double s = h->np ? np_prob(h, 0, 0) + np_prob(h, 0, 1) : device_norm2(h);
         *norm = std::sqrt(s);
@@ -949,7 +973,7 @@ static void validate_terms(qslhip_t h, const qslhip_pauli *terms, uint64_t nterm
 int qslhip_expval(qslhip_t h, const qslhip_pauli *terms, uint64_t nterms, double *value)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(value, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(!h->np, QSLHIP_EKIND, "expectation values need a full-vector handle");
         validate_terms(h, terms, nterms);
@@ -962,7 +986,7 @@ int qslhip_expval_grad(qslhip_t h, const qslhip_op *ops, uint64_t nops,
                        double *grads)
 {
     return guarded([&] {
-        require_handle(h);
+        require_state(h);
         QH_REQUIRE(energy, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
         QH_REQUIRE(!h->np, QSLHIP_EKIND, "gradients need a full-vector handle");

@@ -119,21 +119,26 @@ void run(qslhip_state *h, const std::vector<TilePass> &passes, const TileOp *dop
 }  // namespace
 
 void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                        const std::vector<TileOp> &ops, void *amps, bool exact)
+                        const std::vector<TileOp> &ops, void *amps, bool exact, uint64_t basis)
 {
     if (!amps) amps = h->amps;
+    QH_REQUIRE(basis == qslhip_state::kNoBasis || amps == h->amps, QSLHIP_EINVAL,
+               "a pending basis state belongs to the handle's own vector");
+    const Tuning &tu = tuning();
+    const int threads = h->prec == 8 ? tu.rthreads_f64 : tu.rthreads_f32;
+    bool reg = tu.reg_tile != 0 && h->n >= (unsigned)tu.reg_min_qubits && !ops.empty() &&
+               !(passes.size() == 1 && ops.size() == 1);
+    for (const TilePass &p : passes) reg = reg && reg_tile_ok(p.logt, threads, h->prec);
+    if (reg) {
+        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups, exact, basis);
+        return;
+    }
+    // the other kernels read the state: write the basis state first
+    if (basis != qslhip_state::kNoBasis) launch_fill_basis(h, basis);
     if (passes.empty() || ops.empty()) return;
     if (passes.size() == 1 && ops.size() == 1) {
         if (h->prec == 8) run<double>(h, passes, nullptr, amps, &ops[0]);
         else run<float>(h, passes, nullptr, amps, &ops[0]);
-        return;
-    }
-    const Tuning &tu = tuning();
-    const int threads = h->prec == 8 ? tu.rthreads_f64 : tu.rthreads_f32;
-    bool reg = tu.reg_tile != 0 && h->n >= (unsigned)tu.reg_min_qubits;
-    for (const TilePass &p : passes) reg = reg && reg_tile_ok(p.logt, threads, h->prec);
-    if (reg) {
-        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups, exact);
         return;
     }
     size_t bytes = ops.size() * sizeof(TileOp);

@@ -96,6 +96,11 @@ struct qslhip_state {
     int prec = 8;          // 8 double, 4 float
     uint64_t namps = 0;    // device amplitudes
     void *amps = nullptr;  // device state
+    // A basis state that reset/setBasis recorded but has not written yet
+    // (kNoBasis: `amps` holds the state).  Every reader writes it first; a
+    // register-tile gate pass writes it in place of its first load.
+    static constexpr uint64_t kNoBasis = ~0ull;
+    uint64_t pending_basis = kNoBasis;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     qslhip::DevBuf red;      // reduction partials + result slots
@@ -115,15 +120,17 @@ namespace qslhip {
 void launch_gate(qslhip_state *h, const GateDesc &g, void *amps = nullptr);
 // exact: the reference's per-gate arithmetic (bit-identical to one kernel
 // per gate); false: tolerance-bound fused multiply-adds (QSLHIP_MODE_REORDER)
+// basis: the state is that basis state, not yet written (pending_basis);
+// the first register-tile pass synthesizes it, other paths write it first
 void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                         const std::vector<TileOp> &ops, void *amps = nullptr,
-                        bool exact = true);
+                        bool exact = true, uint64_t basis = qslhip_state::kNoBasis);
 // the same passes with the amplitudes held in registers, `threads` (256 or
 // 512) per workgroup (regtile.hip); reg_tile_ok says whether a kernel
 // exists for the tile width
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                        const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
-                       bool exact);
+                       bool exact, uint64_t basis = qslhip_state::kNoBasis);
 bool reg_tile_ok(int logt, int threads, int prec);
 void launch_fill_basis(qslhip_state *h, uint64_t index);
 double device_prob(qslhip_state *h, unsigned t, unsigned outcome);

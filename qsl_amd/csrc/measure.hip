@@ -6,12 +6,15 @@
 // then one partial per workgroup; a single workgroup adds the partials in a
 // fixed order, so results are deterministic run to run.
 //
-// Sampling (measure.cpp:60-78, 164-188): a three-kernel reduce-then-scan
-// builds the inclusive FP64 CDF over all 2^n amplitudes (zero entries leave
-// the running sum unchanged, so "first index with cdf >= u" never lands on
-// a zero-probability outcome, exactly like generateDist's skip), then one
-// lane per shot binary-searches it.  A uniform above the total mass gives
-// 0, as in drawSample.
+// Sampling (measure.cpp:60-78, 164-188): the outcome of uniform u is the
+// first index whose running FP64 sum of |a|^2 reaches u (zero entries
+// leave the running sum unchanged, so it never lands on a zero-probability
+// outcome, exactly like generateDist's skip).  The running sums are not
+// stored: one pass over the state writes sub-chunk and chunk sums, a
+// single workgroup scans the chunk sums, and one wavefront per shot
+// resolves chunk, sub-chunk and amplitude with 64-lane searches (k_sample).
+// A uniform above the total mass gives 0, as in drawSample.  The compact
+// NP layout keeps a stored CDF (build_cdf_any).
 #include <cstring>
 
 #include "gate_math.h"
@@ -149,6 +152,7 @@ __global__ __launch_bounds__(1024) void k_scan_sums(double *__restrict__ sums, u
         sums[i] = run;
         run += v;
     }
+    if (threadIdx.x == 1023) sums[n] = run;  // the total: the last run's end
 }
 
 // Scan pass 3: inclusive CDF inside each chunk, offset by the chunk prefix.
@@ -194,38 +198,148 @@ __global__ __launch_bounds__(kBlock) void k_chunk_scan(const cplx<T> *__restrict
     }
 }
 
-// First index i with cdf[i] >= u (u > 0), or with cdf[i] > 0 (u <= 0: the
-// reference would index out of bounds; this picks the first outcome of
-// nonzero probability); namps when u exceeds the total -> outcome 0.
-// With `strict`, the first index with cdf[i] > u instead: the merge rule of
-// sampleAll2 (measure.cpp:237-267, `rand < sum`).
-__device__ __forceinline__ uint64_t cdf_search(const double *__restrict__ cdf,
-                                               uint64_t namps, double u, bool strict)
+// ---- sampling without a stored CDF ---------------------------------------
+// A shot's outcome is the first index whose running sum of |a|^2 reaches
+// the uniform, resolved in three levels that each hold 64 entries: chunks
+// of kScanChunk amplitudes (exclusive prefix `pre`, one FP64 per chunk),
+// their 64 sub-chunks of 64 amplitudes (sums `subs`), and the amplitudes
+// of one sub-chunk, re-read from the state.  Only the state is read in
+// full (once, by k_sub_sums); a shot reads 64 prefix values per chunk
+// level and 1 KiB of amplitudes.
+
+constexpr int kSub = 64;                          // amplitudes per sub-chunk
+constexpr int kSubsPerChunk = kScanChunk / kSub;  // 64
+constexpr int kWaves = kBlock / 64;
+constexpr int kSubsPerWave = kSubsPerChunk / kWaves;  // 16
+
+// Sub-chunk sums (one wave reduction each) and chunk sums (their wave
+// totals added in sub order, then the four waves in order).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sub_sums(const cplx<T> *__restrict__ st,
+                                                     uint64_t namps, double *__restrict__ subs,
+                                                     double *__restrict__ sums)
 {
-    uint64_t lo = 0, hi = namps;
-    const bool zero = strict || !(u > 0.0);
-    const double key = strict ? u : 0.0;
-    while (lo < hi) {
-        uint64_t mid = lo + ((hi - lo) >> 1);
-        double v = cdf[mid];
-        bool ge = zero ? (v > key) : (v >= u);
-        if (ge) hi = mid;
-        else lo = mid + 1;
+    __shared__ double sh[kWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t a0 = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)w * kSubsPerWave * kSub;
+    double p[kSubsPerWave];
+#pragma unroll
+    for (int j = 0; j < kSubsPerWave; j++) {
+        const uint64_t i = a0 + (uint64_t)j * kSub + lane;
+        p[j] = i < namps ? sq(st[i]) : 0.0;
     }
-    return lo;
+    double tot = 0, mine = 0;
+#pragma unroll
+    for (int j = 0; j < kSubsPerWave; j++) {
+        const double sj = wave_sum(p[j]);
+        tot += sj;
+        if (lane == j) mine = sj;
+    }
+    if (lane < kSubsPerWave)
+        subs[(uint64_t)blockIdx.x * kSubsPerChunk + w * kSubsPerWave + lane] = mine;
+    if (lane == 0) sh[w] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double c = 0;
+        for (int k = 0; k < kWaves; k++) c += sh[k];
+        sums[blockIdx.x] = c;
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_search(const double *__restrict__ cdf,
+__device__ __forceinline__ double wave_scan_incl(double v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// Whether running sum v has reached the shot: v >= u, or v > key for the
+// strict rule of sampleAll2 (measure.cpp:237-267, `rand < sum`) and for
+// u <= 0 (key 0: the first outcome of nonzero probability, where the
+// reference would index out of bounds).
+struct ShotRule {
+    double u, key;
+    bool gt;
+    __device__ bool operator()(double v) const { return gt ? (v > key) : (v >= u); }
+};
+
+// First lane whose flag is set, or -1.
+__device__ __forceinline__ int first_lane(bool f)
+{
+    const uint64_t b = __ballot(f);
+    return b ? __ffsll((unsigned long long)b) - 1 : -1;
+}
+__device__ __forceinline__ int last_lane(bool f)
+{
+    const uint64_t b = __ballot(f);
+    return b ? 63 - __clzll((long long)b) : -1;
+}
+
+// One wavefront per shot.  Level 1: the first chunk c whose inclusive end
+// pre[c + 1] reaches the shot (64-way search, three rounds for 2^18
+// chunks).  Levels 2 and 3: running sums from pre[c] over the sub-chunk
+// sums, then over the chosen sub-chunk's |a|^2; the outcome is the first
+// entry of nonzero probability whose running sum reaches the shot (zero
+// entries leave the reference's running sum unchanged, and generateDist
+// skips them), else the last entry of nonzero probability (the chunk's
+// total reached the shot, its partial sums fell short by rounding).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sample(const cplx<T> *__restrict__ st,
                                                    uint64_t namps,
+                                                   const double *__restrict__ pre,
+                                                   uint64_t nchunks,
+                                                   const double *__restrict__ subs,
                                                    const double *__restrict__ u,
                                                    uint64_t nshots, int strict,
                                                    uint64_t *__restrict__ out)
 {
-    uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= nshots) return;
-    uint64_t r = cdf_search(cdf, namps, u[i], strict != 0);
+    const int lane = threadIdx.x & 63;
+    const uint64_t shot = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (shot >= nshots) return;  // the whole wave
+    const double x = u[shot];
+    ShotRule hit{x, strict ? x : 0.0, strict != 0 || !(x > 0.0)};
+    // level 1: the answer is in [lo, hi]; hi is a chunk known to hit, or
+    // nchunks (none).  Lane l probes the end of the l-th of 64 equal steps.
+    uint64_t lo = 0, hi = nchunks;
+    while (lo < hi) {
+        const uint64_t span = hi - lo;
+        const uint64_t step = span <= 64 ? 1 : (span + 63) / 64;
+        uint64_t c = lo + (uint64_t)(lane + 1) * step - 1;
+        if (c > hi - 1) c = hi - 1;
+        const int f = first_lane(hit(pre[c + 1]));
+        if (f < 0) {
+            lo = hi;  // the last probe is hi - 1
+        } else {
+            const uint64_t cf = lo + (uint64_t)(f + 1) * step - 1;
+            if (f > 0) lo += (uint64_t)f * step;  // past the probe of lane f - 1
+            hi = cf < hi - 1 ? cf : hi - 1;
+        }
+    }
+    uint64_t c = lo;
+    // levels 2 and 3; a chunk whose partial sums all stay zero (its prefix
+    // moved only by rounding) hands the shot to the next chunk
+    for (; c < nchunks; c++) {
+        const double base = pre[c];
+        const double sv = subs[c * kSubsPerChunk + lane];
+        const double run = base + wave_scan_incl(sv, lane);
+        int j = first_lane(sv > 0.0 && hit(run));
+        if (j < 0) j = last_lane(sv > 0.0);
+        if (j < 0) continue;
+        const double sbase = j ? __shfl(run, j - 1, 64) : base;
+        const uint64_t i = (c * kSubsPerChunk + (uint64_t)j) * kSub + (uint64_t)lane;
+        const double p = i < namps ? sq(st[i]) : 0.0;
+        const double run2 = sbase + wave_scan_incl(p, lane);
+        int e = first_lane(p > 0.0 && hit(run2));
+        if (e < 0) e = last_lane(p > 0.0);
+        if (e < 0) continue;
+        if (lane == 0) out[shot] = (c * kSubsPerChunk + (uint64_t)j) * kSub + (uint64_t)e;
+        return;
+    }
     // past the total mass: drawSample returns 0; sampleAll2 counts nothing
-    out[i] = (r >= namps) ? (strict ? ~0ull : 0ull) : r;
+    if (lane == 0) out[shot] = strict ? ~0ull : 0ull;
 }
 
 template <typename T>
@@ -297,12 +411,23 @@ void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome, double facto
     QH_CHECK(hipGetLastError());
 }
 
+namespace {
+
 // out[i] = basis index sampled by uniform u[i] (host arrays).
-void device_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out,
-                   bool strict)
+template <typename T>
+void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out, bool strict)
 {
-    if (h->prec == 8) build_cdf<double>(h);
-    else build_cdf<float>(h);
+    const uint64_t namps = h->namps;
+    const uint64_t nchunks = (namps + kScanChunk - 1) / kScanChunk;
+    // [chunk prefix: nchunks + 1][sub-chunk sums: 64 per chunk]
+    h->cdf.ensure((nchunks + 1 + nchunks * kSubsPerChunk) * sizeof(double));
+    double *pre = static_cast<double *>(h->cdf.ptr);
+    double *subs = pre + nchunks + 1;
+    const cplx<T> *st = static_cast<const cplx<T> *>(h->amps);
+    hipLaunchKernelGGL(k_sub_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0, h->stream, st,
+                       namps, subs, pre);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, pre, nchunks);
+    QH_CHECK(hipGetLastError());
     if (nshots == 0) {
         QH_CHECK(hipStreamSynchronize(h->stream));
         return;
@@ -312,13 +437,21 @@ void device_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *
     uint64_t *dout = reinterpret_cast<uint64_t *>(du + nshots);
     QH_CHECK(hipMemcpyAsync(du, u, nshots * sizeof(double), hipMemcpyHostToDevice,
                             h->stream));
-    hipLaunchKernelGGL(k_search, dim3(grid_for(nshots, kBlock)), dim3(kBlock), 0,
-                       h->stream, static_cast<const double *>(h->cdf.ptr), h->namps, du,
-                       nshots, (int)strict, dout);
+    hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(nshots, kWaves)), dim3(kBlock), 0, h->stream,
+                       st, namps, pre, nchunks, subs, du, nshots, (int)strict, dout);
     QH_CHECK(hipGetLastError());
     QH_CHECK(hipMemcpyAsync(out, dout, nshots * sizeof(uint64_t), hipMemcpyDeviceToHost,
                             h->stream));
     QH_CHECK(hipStreamSynchronize(h->stream));
+}
+
+}  // namespace
+
+void device_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out,
+                   bool strict)
+{
+    if (h->prec == 8) sample_impl<double>(h, u, nshots, out, strict);
+    else sample_impl<float>(h, u, nshots, out, strict);
 }
 
 }  // namespace qslhip

@@ -64,8 +64,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
     if (g >= ngroups) return;
     uint64_t gbase_n = deposit(g, outside);
     cplx<T> v[R], vn[R];
-#pragma unroll
-    for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];
+    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);
     wait_vmcnt<0>();
     for (;;) {
         const uint64_t gbase = gbase_n;
@@ -75,8 +74,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
         const bool more = gn < ngroups;
         if (more) {
             gbase_n = deposit(gn, outside);
-#pragma unroll
-            for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];
+            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);
             // in flight: the previous group's R stores and these R loads;
             // everything older (this group's loads) is complete
             wait_vmcnt<2 * R>();
@@ -352,6 +350,7 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
 {
     const int K = p.logt, R = 1 << (K - tb);
     RegTileArgs ta{};
+    ta.basis = ~0ull;
     ta.tile_mask = p.tile_mask;
     int pos = 0;
     for (unsigned q = 0; q < 64; q++) {
@@ -380,13 +379,14 @@ namespace {
 
 template <typename T, int NT, int R, bool EX>
 void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dops, uint32_t nops,
-                int gpw)
+                int gpw, uint64_t basis)
 {
     constexpr int A = (int)sizeof(cplx<T>);
     constexpr int tb = NT == 256 ? 8 : 9;
     constexpr int r = R == 4 ? 2 : R == 8 ? 3 : 4;
     const int K = tb + r;
-    const RegTileArgs ta = make_tile_args(h->n, p, tb, A);
+    RegTileArgs ta = make_tile_args(h->n, p, tb, A);
+    ta.basis = basis;
     const size_t shmem = (size_t)A << K;
     const uint64_t groups = 1ull << (h->n - K);
     uint64_t grid;
@@ -411,14 +411,14 @@ void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dop
 
 template <typename T, bool EX>
 void launch_t(qslhip_state *h, void *amps, const TilePass &p, int nt, const RegOp *dops,
-              uint32_t nops, int gpw)
+              uint32_t nops, int gpw, uint64_t basis)
 {
     const int r = p.logt - (nt == 512 ? 9 : 8);
-    if (nt == 256 && r == 2) launch_one<T, 256, 4, EX>(h, amps, p, dops, nops, gpw);
-    else if (nt == 256 && r == 3) launch_one<T, 256, 8, EX>(h, amps, p, dops, nops, gpw);
-    else if (nt == 256 && r == 4) launch_one<T, 256, 16, EX>(h, amps, p, dops, nops, gpw);
-    else if (nt == 512 && r == 3) launch_one<T, 512, 8, EX>(h, amps, p, dops, nops, gpw);
-    else if (nt == 512 && r == 4) launch_one<T, 512, 16, EX>(h, amps, p, dops, nops, gpw);
+    if (nt == 256 && r == 2) launch_one<T, 256, 4, EX>(h, amps, p, dops, nops, gpw, basis);
+    else if (nt == 256 && r == 3) launch_one<T, 256, 8, EX>(h, amps, p, dops, nops, gpw, basis);
+    else if (nt == 256 && r == 4) launch_one<T, 256, 16, EX>(h, amps, p, dops, nops, gpw, basis);
+    else if (nt == 512 && r == 3) launch_one<T, 512, 8, EX>(h, amps, p, dops, nops, gpw, basis);
+    else if (nt == 512 && r == 4) launch_one<T, 512, 16, EX>(h, amps, p, dops, nops, gpw, basis);
     else throw Error(QSLHIP_EINVAL, "no register-tile kernel for this tile width");
 }
 
@@ -434,7 +434,7 @@ bool reg_tile_ok(int logt, int threads, int prec)
 
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                        const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
-                       bool exact)
+                       bool exact, uint64_t basis)
 {
     const int tb = threads == 512 ? 9 : 8;
     const int A = h->prec == 8 ? 16 : 8;
@@ -473,16 +473,17 @@ void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
     for (size_t i = 0; i < passes.size(); i++) {
         const uint32_t n_ops = first[i + 1] - first[i];
         const TilePass &p = passes[i];
+        const uint64_t b = i == 0 ? basis : ~0ull;  // only the first pass reads the state
         if (cbytes && jit_launch_pass(h, amps, p, threads, gpw, exact, rops.data() + first[i],
-                                      n_ops, dcoef + cfirst[i]))
+                                      n_ops, dcoef + cfirst[i], b))
             continue;
         const RegOp *d = dops + first[i];
         if (h->prec == 8) {
-            if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw);
-            else launch_t<double, false>(h, amps, p, threads, d, n_ops, gpw);
+            if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw, b);
+            else launch_t<double, false>(h, amps, p, threads, d, n_ops, gpw, b);
         } else {
-            if (exact) launch_t<float, true>(h, amps, p, threads, d, n_ops, gpw);
-            else launch_t<float, false>(h, amps, p, threads, d, n_ops, gpw);
+            if (exact) launch_t<float, true>(h, amps, p, threads, d, n_ops, gpw, b);
+            else launch_t<float, false>(h, amps, p, threads, d, n_ops, gpw, b);
         }
     }
 }

@@ -35,7 +35,28 @@ struct RegTileArgs {
     uint64_t tlow;               // the tb lowest of them (thread bits)
     uint64_t kq[4];              // qubit of tile position tb + i, as a one-bit mask
     uint32_t koff0[kRegMaxR];    // swizzled LDS byte offsets of the load layout
+    uint64_t basis;              // ~0: load the state; else the state is this
+                                 // basis state, synthesized without a read
 };
+
+// One thread's R amplitudes of a group: state indices idx0 | kpart[k].  With
+// a basis index the state is |basis>, never written to memory: 1 + 0i at
+// that index and +0 elsewhere, the values a fill of the vector would hold.
+template <typename T, int R>
+__device__ __forceinline__ void reg_load(cplx<T> (&v)[R], const cplx<T> *st, uint64_t idx0,
+                                         const uint64_t (&kpart)[R], uint64_t basis)
+{
+    if (basis == ~0ull) {
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = st[idx0 | kpart[k]];
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            v[k].re = (idx0 | kpart[k]) == basis ? T(1) : T(0);
+            v[k].im = T(0);
+        }
+    }
+}
 
 // rotateZ's two diagonal halves without a branch on the bit: the |1> half
 // is the |0> half with s negated (x + y*(-s) is x - y*s exactly in IEEE

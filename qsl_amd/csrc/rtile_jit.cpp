@@ -81,8 +81,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "    if (g >= ngroups) return;\n"
          "    uint64_t gbase_n = deposit(g, outside);\n"
          "    cplx<T> v[R], vn[R];\n"
-         "#pragma unroll\n"
-         "    for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];\n"
+         "    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);\n"
          "    wait_vmcnt<0>();\n"
          "    for (;;) {\n"
          "        const uint64_t gbase = gbase_n;\n"
@@ -92,8 +91,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "        const bool more = gn < ngroups;\n"
          "        if (more) {\n"
          "            gbase_n = deposit(gn, outside);\n"
-         "#pragma unroll\n"
-         "            for (int k = 0; k < R; k++) vn[k] = st[gbase_n | tthread | kpart[k]];\n"
+         "            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);\n"
          "            wait_vmcnt<2 * R>();\n"
          "        } else {\n"
          "            wait_vmcnt<R>();\n"
@@ -253,7 +251,8 @@ void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef
 }
 
 bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads, int gpw,
-                     bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef)
+                     bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef,
+                     uint64_t basis)
 {
     const int mode = tuning().jit;
     if (mode == 0) return false;
@@ -289,6 +288,7 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     void *st = amps;
     unsigned n = h->n;
     RegTileArgs targ = ta;
+    targ.basis = basis;
     const double *cf = dev_coef;
     unsigned long long ng = groups;
     void *args[] = {&st, &n, &targ, &cf, &ng};

@@ -1,0 +1,140 @@
+"""Basis states written lazily: reset(), setBasisState() and measureAll()
+record the basis index; the next reader writes the vector, and the next
+register-tile gate pass synthesizes the amplitudes in place of its first
+load.  Every observable must be bit-identical to a state written eagerly
+(setState of the same basis vector) -- default.cpp:57-65 (reset),
+default.cpp:76-82 (setBasisState).
+"""
+import numpy as np
+import pytest
+
+import qsl_amd
+from qsl_amd import _lib as L
+from oracle.pyoracle import Oracle, OP_DTYPE, GATE1, GATE2
+
+pytestmark = pytest.mark.gpu
+
+O = Oracle()
+
+
+def _ops(rng, n, count, dtype):
+    kinds = GATE1 + GATE2
+    ops = np.zeros(count, OP_DTYPE)
+    for i in range(count):
+        k = int(rng.choice(kinds))
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        ang = float(rng.uniform(-2 * np.pi, 2 * np.pi))
+        if dtype == np.complex64:
+            ang = float(np.float32(ang))
+        ops[i] = (k, a, b if k >= 16 else 0, 0, ang)
+    return ops
+
+
+def _basis(n, idx, dtype):
+    v = np.zeros(1 << n, dtype)
+    v[idx] = 1
+    return v
+
+
+def _same(a, b):
+    return np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.fixture
+def jit_mode():
+    saved = L.tuning_get("jit")
+    yield lambda v: L.tuning_set("jit", v)
+    L.tuning_set("jit", saved)
+
+
+MODES = [("fused", qsl_amd.MODE_FUSED), ("reorder", qsl_amd.MODE_REORDER),
+         ("per_gate", qsl_amd.MODE_PER_GATE)]
+
+
+@pytest.mark.parametrize("jit", [0, 2])
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: m[0])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [12, 20])
+def test_gates_on_a_recorded_basis_state(jit_mode, jit, mode, dtype, n):
+    # n = 20 takes the register-tile passes (synthesized first load), n = 12
+    # the LDS tile kernel (the basis state is written first)
+    jit_mode(jit)
+    rng = np.random.default_rng(n * 10 + jit + (mode[1] << 4))
+    ops = _ops(rng, n, 200, dtype)
+    for idx in (0, int(rng.integers(1, 1 << n))):
+        lazy = qsl_amd.Qubits(n, dtype=dtype)
+        if idx:
+            lazy.setBasisState(idx)
+        lazy.apply_ops(ops, mode=mode[1])
+        eager = qsl_amd.Qubits(n, dtype=dtype)
+        eager.setState(_basis(n, idx, dtype))
+        eager.apply_ops(ops, mode=mode[1])
+        got = lazy.getState()
+        assert _same(got, eager.getState()), (mode[0], idx)
+        if mode[1] != qsl_amd.MODE_REORDER:
+            want = _basis(n, idx, dtype)
+            O.apply_ops(want, ops)
+            assert _same(got, want), (mode[0], idx)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_readers_see_the_recorded_state(dtype):
+    n = 20
+    q = qsl_amd.Qubits(n, dtype=dtype)
+    rng = np.random.default_rng(7)
+    q.apply_ops(_ops(rng, n, 50, dtype), mode=qsl_amd.MODE_FUSED)
+    # reset after gates: every reader sees |0...0>
+    q.reset()
+    assert q.prob(0, 1) == 0.0 and q.prob(5, 0) == 1.0
+    assert q.norm() == 1.0
+    assert _same(q.getState(), _basis(n, 0, dtype))
+    idx = 0b1011_0000_0000_0110_1001
+    q.setBasisState(idx)
+    amps = q.getAmps(idx - 2, 5)
+    assert _same(amps, np.array([0, 0, 1, 0, 0], dtype))
+    q.setBasisState(idx)
+    assert all(q.prob(t, (idx >> t) & 1) == 1.0 for t in range(n))
+    q.setBasisState(idx)
+    c = q.copy()
+    assert _same(c.getState(), _basis(n, idx, dtype))
+    q.setBasisState(3)
+    q.setAmps(np.array([0.5, 0.5], dtype), first=1)  # a partial write keeps the rest
+    want = _basis(n, 3, dtype)
+    want[1:3] = 0.5
+    assert _same(q.getState(), want)
+    # assign from a handle holding a recorded state, and onto one
+    a = qsl_amd.Qubits(n, dtype=dtype)
+    a.setBasisState(9)
+    b = qsl_amd.Qubits(n, dtype=dtype)
+    b.assign(a)
+    assert _same(b.getState(), _basis(n, 9, dtype))
+    b.setBasisState(4)
+    b.assign(a)
+    assert _same(b.getState(), _basis(n, 9, dtype))
+
+
+def test_measure_all_then_gates():
+    # measureAll collapses onto the outcome (measure.cpp:110-123); the
+    # collapsed basis state is recorded and synthesized by the next pass
+    n = 20
+    rng = np.random.default_rng(11)
+    q = qsl_amd.Qubits(n)
+    q.apply_ops(_ops(rng, n, 100, np.complex128), mode=qsl_amd.MODE_FUSED)
+    o = q.measureAll(0.37)
+    assert _same(q.getState(), _basis(n, o, np.complex128))
+    o2 = q.measureAll(0.61)
+    assert o2 == o
+    ops = _ops(rng, n, 150, np.complex128)
+    q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    want = _basis(n, o, np.complex128)
+    O.apply_ops(want, ops)
+    assert _same(q.getState(), want)
+
+
+def test_empty_op_list_keeps_the_recorded_state():
+    n = 20
+    q = qsl_amd.Qubits(n)
+    q.setBasisState(5)
+    q.apply_ops(np.zeros(0, OP_DTYPE), mode=qsl_amd.MODE_FUSED)
+    q.apply_ops(np.array([(0, 3, 0, 0, 0.0)], OP_DTYPE), mode=qsl_amd.MODE_FUSED)  # pauliX
+    assert _same(q.getState(), _basis(n, 5 ^ 8, np.complex128))
