@@ -109,6 +109,7 @@ void reset_impl(qslhip_t h)
     // lowest weight-k string 2^k - 1, whose colex rank is 0 (np.cpp:68-79).
     if (h->np) launch_fill_basis(h, 0);
     else h->pending_basis = 0;
+    h->sums_valid = false;
 }
 
 // Writes a recorded basis state into the vector, before anything reads it.
@@ -119,11 +120,20 @@ void materialize(qslhip_t h)
     h->pending_basis = qslhip_state::kNoBasis;
 }
 
-// A handle whose vector holds its state.
-void require_state(qslhip_t h)
+// A handle whose vector holds its state, for an entry point that only
+// reads it.
+void require_state_ro(qslhip_t h)
 {
     require_handle(h);
     materialize(h);
+}
+
+// The same for one that changes the state: the sampling sums a gate pass
+// left no longer describe it.
+void require_state(qslhip_t h)
+{
+    require_state_ro(h);
+    h->sums_valid = false;
 }
 
 template <typename Fp>
@@ -296,7 +306,7 @@ int qslhip_destroy(qslhip_t h)
 int qslhip_clone(qslhip_t src, qslhip_t *out)
 {
     return guarded([&] {
-        require_state(src);
+        require_state_ro(src);
         QH_REQUIRE(out, QSLHIP_EINVAL, "null output handle");
         *out = nullptr;
         qslhip_t h = alloc_handle(src->n, src->prec, src->np, src->k);
@@ -312,9 +322,10 @@ int qslhip_assign(qslhip_t dst, qslhip_t src)
 {
     return guarded([&] {
         require_handle(dst);
-        require_state(src);
+        require_state_ro(src);
         if (dst == src) return;
         dst->pending_basis = qslhip_state::kNoBasis;  // overwritten
+        dst->sums_valid = false;
         // default.cpp:94-105 / np.cpp:119-131
         QH_REQUIRE(dst->n == src->n, QSLHIP_ESIZE,
                    "Cannot assign Qubits object vector from different number of qubits");
@@ -391,6 +402,7 @@ int qslhip_set_state(qslhip_t h, const void *host, uint64_t count)
         // default.cpp:70-74
         QH_REQUIRE(count == (1ull << h->n), QSLHIP_ESIZE,
                    "Cannot assign state vector from different number of qubits");
+        h->sums_valid = false;
         if (!h->np) {
             h->pending_basis = qslhip_state::kNoBasis;  // overwritten
             QH_CHECK(hipMemcpyAsync(h->amps, host, count * h->amp_bytes(),
@@ -427,7 +439,7 @@ int qslhip_set_state(qslhip_t h, const void *host, uint64_t count)
 int qslhip_get_state(qslhip_t h, void *host, uint64_t count)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(host, QSLHIP_EINVAL, "null host pointer");
         QH_REQUIRE(count == (1ull << h->n), QSLHIP_ESIZE, "count must be 2^nqubits");
         if (!h->np) {
@@ -449,7 +461,7 @@ int qslhip_get_state(qslhip_t h, void *host, uint64_t count)
 int qslhip_get_amps(qslhip_t h, void *host, uint64_t first, uint64_t count)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(host || count == 0, QSLHIP_EINVAL, "null host pointer");
         QH_REQUIRE(first <= h->namps && count <= h->namps - first, QSLHIP_EINVAL,
                    "amplitude range out of bounds");
@@ -481,6 +493,7 @@ int qslhip_set_basis(qslhip_t h, uint64_t index)
         require_handle(h);
         QH_REQUIRE(h->n >= 64 || index < (1ull << h->n), QSLHIP_EINVAL,
                    "basis index out of range");
+        h->sums_valid = false;
         if (!h->np) {
             h->pending_basis = index;  // written by the next reader or gate pass
             return;
@@ -581,6 +594,7 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
     return guarded([&] {
         require_handle(h);
         QH_REQUIRE(ops || nops == 0, QSLHIP_EINVAL, "null op list");
+        h->sums_valid = false;
         QH_REQUIRE(mode == QSLHIP_MODE_PER_GATE || mode == QSLHIP_MODE_FUSED ||
                        mode == QSLHIP_MODE_DENSE || mode == QSLHIP_MODE_REORDER,
                    QSLHIP_EINVAL, "unknown mode");
@@ -632,10 +646,17 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
         } else {
             plan_tile_passes(h->n, h->prec, gates, logt, low, passes, tops);
         }
-        // a recorded basis state is synthesized by the first pass (or written first)
-        launch_tile_passes(h, passes, tops, nullptr, mode != QSLHIP_MODE_REORDER,
-                           h->pending_basis);
+        // a recorded basis state is synthesized by the first pass (or written
+        // first); the last pass leaves the sampling sums of the result
+        const bool sums = launch_tile_passes(h, passes, tops, nullptr,
+                                             mode != QSLHIP_MODE_REORDER, h->pending_basis,
+                                             sample_run_sums(h));
         h->pending_basis = qslhip_state::kNoBasis;
+        h->sums_valid = sums;
+        if (sums) {
+            h->sums_tile = passes.back().tile_mask;
+            h->sums_logt = passes.back().logt;
+        }
     });
 }
 
@@ -817,7 +838,7 @@ int qslhip_plan_dense(unsigned nqubits, const qslhip_op *ops, uint64_t nops, int
 int qslhip_prob(qslhip_t h, unsigned targ, unsigned outcome, double *p)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(p, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
         *p = h->np ? np_prob(h, targ, outcome) : device_prob(h, targ, outcome);
@@ -884,7 +905,7 @@ int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p)
 int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots, uint64_t *out)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE((u && out) || nshots == 0, QSLHIP_EINVAL, "null array");
         std::vector<double> uf;
         const double *src = u;
@@ -902,7 +923,7 @@ int qslhip_sample_all2(qslhip_t h, const double *u, uint64_t nshots,
                        uint64_t *counts, uint64_t ncounts)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(counts && (u || nshots == 0), QSLHIP_EINVAL, "null array");
         QH_REQUIRE(ncounts == (1ull << h->n), QSLHIP_ESIZE, "ncounts must be 2^nqubits");
         std::vector<double> uf(u, u + nshots);
@@ -920,7 +941,7 @@ int qslhip_sample_all2(qslhip_t h, const double *u, uint64_t nshots,
 int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(outcome, QSLHIP_EINVAL, "null output");
         double uu = as_fp(h, u);
         uint64_t o = 0;
@@ -929,6 +950,7 @@ int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome)
         // measure.cpp:110-123: collapse onto the outcome
         if (h->np) launch_fill_basis(h, np_rank_host(o));
         else h->pending_basis = o;  // the collapsed state, written when next read
+        h->sums_valid = false;
         QH_CHECK(hipStreamSynchronize(h->stream));
         *outcome = o;
     });
@@ -938,7 +960,7 @@ int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
                   uint64_t *counts)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(counts && (u || nshots == 0), QSLHIP_EINVAL, "null array");
         QH_REQUIRE(targ < h->n, QSLHIP_EINVAL, "qubit index out of range");
         // measure.cpp:191-216: one prob, then u < prob0 counts as 0
@@ -954,7 +976,7 @@ int qslhip_sample(qslhip_t h, unsigned targ, const double *u, uint64_t nshots,
 int qslhip_norm(qslhip_t h, double *norm)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(norm, QSLHIP_EINVAL, "null output");
         double s = h->np ? np_prob(h, 0, 0) + np_prob(h, 0, 1) : device_norm2(h);
         *norm = std::sqrt(s);
@@ -973,7 +995,7 @@ static void validate_terms(qslhip_t h, const qslhip_pauli *terms, uint64_t nterm
 int qslhip_expval(qslhip_t h, const qslhip_pauli *terms, uint64_t nterms, double *value)
 {
     return guarded([&] {
-        require_state(h);
+        require_state_ro(h);
         QH_REQUIRE(value, QSLHIP_EINVAL, "null output");
         QH_REQUIRE(!h->np, QSLHIP_EKIND, "expectation values need a full-vector handle");
         validate_terms(h, terms, nterms);

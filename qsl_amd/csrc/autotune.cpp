@@ -111,6 +111,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "rgroups") *value = t.rgroups;
         else if (k == "reg_min_qubits") *value = t.reg_min_qubits;
         else if (k == "jit") *value = t.jit;
+        else if (k == "jit_waves") *value = t.jit_waves;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
         else throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
@@ -156,6 +157,9 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "jit") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit must be 0, 1 or 2");
             t.jit = value;
+        } else if (k == "jit_waves") {
+            QH_REQUIRE(value >= 0 && value <= 8, QSLHIP_EINVAL, "jit_waves must be in [0, 8]");
+            t.jit_waves = value;
         } else if (k == "reg_min_qubits") {
             QH_REQUIRE(value >= 0 && value <= (int)kMaxQubits, QSLHIP_EINVAL,
                        "reg_min_qubits must be in [0, 40]");

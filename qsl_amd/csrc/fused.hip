@@ -118,8 +118,9 @@ void run(qslhip_state *h, const std::vector<TilePass> &passes, const TileOp *dop
 
 }  // namespace
 
-void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
-                        const std::vector<TileOp> &ops, void *amps, bool exact, uint64_t basis)
+bool launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
+                        const std::vector<TileOp> &ops, void *amps, bool exact, uint64_t basis,
+                        double *runs)
 {
     if (!amps) amps = h->amps;
     QH_REQUIRE(basis == qslhip_state::kNoBasis || amps == h->amps, QSLHIP_EINVAL,
@@ -130,16 +131,20 @@ void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                !(passes.size() == 1 && ops.size() == 1);
     for (const TilePass &p : passes) reg = reg && reg_tile_ok(p.logt, threads, h->prec);
     if (reg) {
-        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups, exact, basis);
-        return;
+        // run sums need qubits 0..2 on the lanes of the last pass's layout,
+        // and eight groups for the sampler to regroup them (k_subs_from_runs)
+        if (runs && ((passes.back().tile_mask & 7) != 7 || h->n < (unsigned)passes.back().logt + 3))
+            runs = nullptr;
+        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups, exact, basis, runs);
+        return runs != nullptr;
     }
     // the other kernels read the state: write the basis state first
     if (basis != qslhip_state::kNoBasis) launch_fill_basis(h, basis);
-    if (passes.empty() || ops.empty()) return;
+    if (passes.empty() || ops.empty()) return false;
     if (passes.size() == 1 && ops.size() == 1) {
         if (h->prec == 8) run<double>(h, passes, nullptr, amps, &ops[0]);
         else run<float>(h, passes, nullptr, amps, &ops[0]);
-        return;
+        return false;
     }
     size_t bytes = ops.size() * sizeof(TileOp);
     // The pinned staging buffer may still be read by the previous copy.
@@ -153,6 +158,7 @@ void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
     const TileOp *dops = static_cast<const TileOp *>(h->optab.ptr);
     if (h->prec == 8) run<double>(h, passes, dops, amps, nullptr);
     else run<float>(h, passes, dops, amps, nullptr);
+    return false;
 }
 
 }  // namespace qslhip

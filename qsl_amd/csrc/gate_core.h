@@ -264,4 +264,33 @@ __device__ __forceinline__ uint64_t ins0(uint64_t x, unsigned p)
     return ((x ^ lo) << 1) | lo;
 }
 
+// |a|^2 formed in Fp, as the reference's `Fp prob = re*re + im*im`
+// (src/qubits/default/measure.cpp:66-67), then widened to FP64 for sums.
+template <typename T>
+__device__ __forceinline__ double amp_prob(const cplx<T> &a)
+{
+    T p = a.re * a.re + a.im * a.im;
+    return (double)p;
+}
+
+// Sum over the 64 lanes of a wavefront, xor butterfly: every lane gets the
+// same total, and the association depends only on the lane order.
+__device__ __forceinline__ double wave_sum_xor(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Xor butterfly over lane bits below `width` (1 << steps lanes), lowest bit
+// first: the sum of 8 lanes is ((l0+l1)+(l2+l3))+((l4+l5)+(l6+l7)), and
+// steps 6 extends that tree over the eight 8-lane sums.
+template <int STEPS>
+__device__ __forceinline__ double lane_sum_up(double v)
+{
+#pragma unroll
+    for (int i = 0; i < STEPS; i++) v += __shfl_xor(v, 1 << i, 64);
+    return v;
+}
+
 }  // namespace qslhip

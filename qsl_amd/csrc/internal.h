@@ -101,6 +101,12 @@ struct qslhip_state {
     // register-tile gate pass writes it in place of its first load.
     static constexpr uint64_t kNoBasis = ~0ull;
     uint64_t pending_basis = kNoBasis;
+    // The sampling sums of 8-amplitude runs (sample_run_sums) hold the
+    // current state: a gate pass wrote them, in the group order of its tile
+    // (sums_tile, 2^sums_logt amplitudes), and nothing has changed it since.
+    bool sums_valid = false;
+    uint64_t sums_tile = 0;
+    int sums_logt = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     qslhip::DevBuf red;      // reduction partials + result slots
@@ -121,22 +127,29 @@ void launch_gate(qslhip_state *h, const GateDesc &g, void *amps = nullptr);
 // exact: the reference's per-gate arithmetic (bit-identical to one kernel
 // per gate); false: tolerance-bound fused multiply-adds (QSLHIP_MODE_REORDER)
 // basis: the state is that basis state, not yet written (pending_basis);
-// the first register-tile pass synthesizes it, other paths write it first
-void launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
+// the first register-tile pass synthesizes it, other paths write it first.
+// runs: where the last register-tile pass may write the sampling sums of
+// the result (sample_run_sums); returns whether it did.
+bool launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                         const std::vector<TileOp> &ops, void *amps = nullptr,
-                        bool exact = true, uint64_t basis = qslhip_state::kNoBasis);
+                        bool exact = true, uint64_t basis = qslhip_state::kNoBasis,
+                        double *runs = nullptr);
 // the same passes with the amplitudes held in registers, `threads` (256 or
 // 512) per workgroup (regtile.hip); reg_tile_ok says whether a kernel
 // exists for the tile width
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                        const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
-                       bool exact, uint64_t basis = qslhip_state::kNoBasis);
+                       bool exact, uint64_t basis = qslhip_state::kNoBasis,
+                       double *runs = nullptr);
 bool reg_tile_ok(int logt, int threads, int prec);
 void launch_fill_basis(qslhip_state *h, uint64_t index);
 double device_prob(qslhip_state *h, unsigned t, unsigned outcome);
 double device_norm2(qslhip_state *h);
 void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome,
                      double factor);
+// where a gate pass writes the sampling sums of the state's 8-amplitude
+// runs (null when the state is too small for the sampler's layout)
+double *sample_run_sums(qslhip_state *h);
 void device_sample(qslhip_state *h, const double *u, uint64_t nshots,
                    uint64_t *out, bool strict = false);
 
@@ -185,6 +198,8 @@ struct Tuning {
     int reg_min_qubits = 18;     // smaller states: LDS tiles (latency bound there)
     int jit = 1;                 // register-tile passes compiled at run time: 0 never,
                                  // 1 once a pass structure repeats, 2 always
+    int jit_waves = 0;           // compiled passes: register budget for this many
+                                 // waves per SIMD (0: the compiler's choice)
 };
 Tuning &tuning();
 extern int g_jit_compiled, g_jit_failed;  // passes compiled at run time / failed

@@ -27,12 +27,7 @@ constexpr int kRedBlocks = 2048;       // partials of a reduction
 constexpr int kScanItems = 16;         // amplitudes per lane in the scan
 constexpr int kScanChunk = kBlock * kScanItems;
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_xor(v); }
 
 // Block sum; result valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double *sh)
@@ -50,8 +45,7 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 template <typename T>
 __device__ __forceinline__ double sq(const cplx<T> &a)
 {
-    T p = a.re * a.re + a.im * a.im;  // in Fp, like the reference
-    return (double)p;
+    return amp_prob(a);  // in Fp, like the reference
 }
 
 // Sum of |a|^2 over amplitudes with bit t == outcome (t < 0: all).
@@ -231,7 +225,9 @@ __global__ __launch_bounds__(kBlock) void k_sub_sums(const cplx<T> *__restrict__
     double tot = 0, mine = 0;
 #pragma unroll
     for (int j = 0; j < kSubsPerWave; j++) {
-        const double sj = wave_sum(p[j]);
+        // 8-amplitude runs first, then the runs: the association a gate
+        // pass's run sums (reg_run_sums) and k_subs_from_runs reproduce
+        const double sj = lane_sum_up<6>(p[j]);
         tot += sj;
         if (lane == j) mine = sj;
     }
@@ -244,6 +240,68 @@ __global__ __launch_bounds__(kBlock) void k_sub_sums(const cplx<T> *__restrict__
         for (int k = 0; k < kWaves; k++) c += sh[k];
         sums[blockIdx.x] = c;
     }
+}
+
+// Where the run sums of a gate pass sit (reg_run_sums): runs[g * 2^lb + l]
+// holds the run of tile index l << 3 in group g, lb = K - 3.
+struct RunLayout {
+    uint64_t tile, outside;  // the pass's tile qubits, the others
+    uint32_t other;          // block bits that index a block's sub-chunks
+    uint32_t qbit[3];        // block bit of qubit 3, 4, 5
+    int lb;
+};
+
+// Sub-chunk sums from a gate pass's run sums.  Workgroup b takes groups
+// 8b..8b+7 into LDS in order: block bits 0..lb-1 are a run's tile bits
+// (tile qubits above 2, ascending), bits lb..lb+2 the group's three lowest
+// bits (the three lowest outside qubits).  Qubits 3..5 are among them, so
+// the eight runs of a sub-chunk are all in the block, combined by the tree
+// lane_sum_up<6> builds over them in k_sub_sums; the other lb bits
+// enumerate the block's 2^lb sub-chunks.
+__global__ __launch_bounds__(kBlock) void k_subs_from_runs(const double *__restrict__ runs,
+                                                           RunLayout L,
+                                                           double *__restrict__ subs)
+{
+    extern __shared__ double blk[];
+    const uint32_t nb = 8u << L.lb;
+    const double *src = runs + (uint64_t)blockIdx.x * nb;
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock) blk[i] = src[i];
+    __syncthreads();
+    const uint32_t lmask = (1u << L.lb) - 1;
+    for (uint32_t u = threadIdx.x; u <= lmask; u += kBlock) {
+        const uint32_t p0 = (uint32_t)deposit(u, L.other);
+        double r[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            uint32_t p = p0;
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+                if ((k >> b) & 1) p |= 1u << L.qbit[b];
+            r[k] = blk[p];
+        }
+        const double s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        const uint64_t g = (uint64_t)blockIdx.x * 8 + (p0 >> L.lb);
+        const uint64_t a = deposit(g, L.outside) | deposit((uint64_t)(p0 & lmask) << 3, L.tile);
+        subs[a >> 6] = s;
+    }
+}
+
+// Chunk sums from sub-chunk sums: the same order of additions as
+// k_sub_sums (a wave's 16 in order, then the four waves).
+__global__ __launch_bounds__(kBlock) void k_chunk_from_subs(const double *__restrict__ subs,
+                                                            uint64_t nchunks,
+                                                            double *__restrict__ sums)
+{
+    const uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nchunks) return;
+    const double *sc = subs + c * kSubsPerChunk;
+    double tot = 0;
+    for (int w = 0; w < kWaves; w++) {
+        double tw = 0;
+        for (int j = 0; j < kSubsPerWave; j++) tw += sc[w * kSubsPerWave + j];
+        tot += tw;
+    }
+    sums[c] = tot;
 }
 
 __device__ __forceinline__ double wave_scan_incl(double v, int lane)
@@ -413,19 +471,70 @@ void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome, double facto
 
 namespace {
 
+// [chunk prefix: nchunks + 1][sub-chunk sums: 64 per chunk][8-amplitude
+// run sums: 8 per sub-chunk]; returns the sub-chunk sums
+double *run_sums_buffer(qslhip_state *h)
+{
+    const uint64_t nchunks = (h->namps + kScanChunk - 1) / kScanChunk;
+    const uint64_t nsubs = nchunks * kSubsPerChunk;
+    h->cdf.ensure((nchunks + 1 + nsubs + nsubs * 8) * sizeof(double));
+    return static_cast<double *>(h->cdf.ptr) + nchunks + 1;
+}
+
+RunLayout run_layout(const qslhip_state *h)
+{
+    const int K = h->sums_logt;
+    QH_REQUIRE((h->sums_tile & 7) == 7 && __builtin_popcountll(h->sums_tile) == K &&
+                   h->n >= (unsigned)K + 3 && K >= 6 && K <= 13,
+               QSLHIP_EINVAL, "run sums from an unexpected tile");
+    RunLayout L{};
+    L.tile = h->sums_tile;
+    L.outside = ((h->n >= 64) ? ~0ull : ((1ull << h->n) - 1)) & ~L.tile;
+    L.lb = K - 3;
+    int bit[6] = {-1, -1, -1, -1, -1, -1};  // block bit of qubits 0..5
+    int lpos = 0;
+    for (unsigned q = 3; q < h->n && lpos < L.lb; q++)
+        if ((L.tile >> q) & 1) {
+            if (q < 6) bit[q] = lpos;
+            lpos++;
+        }
+    int opos = 0;
+    for (unsigned q = 3; q < h->n && opos < 3; q++)
+        if ((L.outside >> q) & 1) {
+            if (q < 6) bit[q] = L.lb + opos;
+            opos++;
+        }
+    uint32_t all = (1u << (L.lb + 3)) - 1, m345 = 0;
+    for (int b = 0; b < 3; b++) {
+        QH_REQUIRE(bit[3 + b] >= 0, QSLHIP_EINVAL, "qubit 3..5 outside the run block");
+        L.qbit[b] = (uint32_t)bit[3 + b];
+        m345 |= 1u << bit[3 + b];
+    }
+    L.other = all & ~m345;
+    return L;
+}
+
 // out[i] = basis index sampled by uniform u[i] (host arrays).
 template <typename T>
 void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out, bool strict)
 {
     const uint64_t namps = h->namps;
     const uint64_t nchunks = (namps + kScanChunk - 1) / kScanChunk;
-    // [chunk prefix: nchunks + 1][sub-chunk sums: 64 per chunk]
-    h->cdf.ensure((nchunks + 1 + nchunks * kSubsPerChunk) * sizeof(double));
-    double *pre = static_cast<double *>(h->cdf.ptr);
-    double *subs = pre + nchunks + 1;
+    double *subs = run_sums_buffer(h);
+    double *pre = subs - (nchunks + 1);
     const cplx<T> *st = static_cast<const cplx<T> *>(h->amps);
-    hipLaunchKernelGGL(k_sub_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0, h->stream, st,
-                       namps, subs, pre);
+    if (h->sums_valid) {  // the last gate pass wrote the run sums
+        const uint64_t nsubs = nchunks * kSubsPerChunk;
+        const RunLayout L = run_layout(h);
+        const uint64_t nblocks = 1ull << (h->n - h->sums_logt - 3);
+        hipLaunchKernelGGL(k_subs_from_runs, dim3((unsigned)nblocks), dim3(kBlock),
+                           (size_t)64 << L.lb, h->stream, subs + nsubs, L, subs);
+        hipLaunchKernelGGL(k_chunk_from_subs, dim3(grid_for(nchunks, kBlock)), dim3(kBlock), 0,
+                           h->stream, subs, nchunks, pre);
+    } else {
+        hipLaunchKernelGGL(k_sub_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0, h->stream,
+                           st, namps, subs, pre);
+    }
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, pre, nchunks);
     QH_CHECK(hipGetLastError());
     if (nshots == 0) {
@@ -446,6 +555,14 @@ void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *ou
 }
 
 }  // namespace
+
+double *sample_run_sums(qslhip_state *h)
+{
+    // whole chunks only: every sub-chunk sum k_chunk_from_subs reads is written
+    if (h->np || h->namps % kScanChunk) return nullptr;
+    const uint64_t nsubs = (h->namps / kScanChunk) * kSubsPerChunk;
+    return run_sums_buffer(h) + nsubs;
+}
 
 void device_sample(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out,
                    bool strict)

@@ -185,6 +185,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
             }
             }
         }
+        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);
 #pragma unroll
         for (int k = 0; k < R; k++) st[gbase | tthread | kpart[k]] = v[k];
         if (!more) break;
@@ -351,6 +352,7 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
     const int K = p.logt, R = 1 << (K - tb);
     RegTileArgs ta{};
     ta.basis = ~0ull;
+    ta.runs = nullptr;
     ta.tile_mask = p.tile_mask;
     int pos = 0;
     for (unsigned q = 0; q < 64; q++) {
@@ -379,7 +381,7 @@ namespace {
 
 template <typename T, int NT, int R, bool EX>
 void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dops, uint32_t nops,
-                int gpw, uint64_t basis)
+                int gpw, uint64_t basis, double *runs)
 {
     constexpr int A = (int)sizeof(cplx<T>);
     constexpr int tb = NT == 256 ? 8 : 9;
@@ -387,6 +389,7 @@ void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dop
     const int K = tb + r;
     RegTileArgs ta = make_tile_args(h->n, p, tb, A);
     ta.basis = basis;
+    ta.runs = runs;
     const size_t shmem = (size_t)A << K;
     const uint64_t groups = 1ull << (h->n - K);
     uint64_t grid;
@@ -411,14 +414,14 @@ void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dop
 
 template <typename T, bool EX>
 void launch_t(qslhip_state *h, void *amps, const TilePass &p, int nt, const RegOp *dops,
-              uint32_t nops, int gpw, uint64_t basis)
+              uint32_t nops, int gpw, uint64_t basis, double *runs)
 {
     const int r = p.logt - (nt == 512 ? 9 : 8);
-    if (nt == 256 && r == 2) launch_one<T, 256, 4, EX>(h, amps, p, dops, nops, gpw, basis);
-    else if (nt == 256 && r == 3) launch_one<T, 256, 8, EX>(h, amps, p, dops, nops, gpw, basis);
-    else if (nt == 256 && r == 4) launch_one<T, 256, 16, EX>(h, amps, p, dops, nops, gpw, basis);
-    else if (nt == 512 && r == 3) launch_one<T, 512, 8, EX>(h, amps, p, dops, nops, gpw, basis);
-    else if (nt == 512 && r == 4) launch_one<T, 512, 16, EX>(h, amps, p, dops, nops, gpw, basis);
+    if (nt == 256 && r == 2) launch_one<T, 256, 4, EX>(h, amps, p, dops, nops, gpw, basis, runs);
+    else if (nt == 256 && r == 3) launch_one<T, 256, 8, EX>(h, amps, p, dops, nops, gpw, basis, runs);
+    else if (nt == 256 && r == 4) launch_one<T, 256, 16, EX>(h, amps, p, dops, nops, gpw, basis, runs);
+    else if (nt == 512 && r == 3) launch_one<T, 512, 8, EX>(h, amps, p, dops, nops, gpw, basis, runs);
+    else if (nt == 512 && r == 4) launch_one<T, 512, 16, EX>(h, amps, p, dops, nops, gpw, basis, runs);
     else throw Error(QSLHIP_EINVAL, "no register-tile kernel for this tile width");
 }
 
@@ -434,10 +437,12 @@ bool reg_tile_ok(int logt, int threads, int prec)
 
 void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                        const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
-                       bool exact, uint64_t basis)
+                       bool exact, uint64_t basis, double *runs)
 {
     const int tb = threads == 512 ? 9 : 8;
     const int A = h->prec == 8 ? 16 : 8;
+    QH_REQUIRE(!runs || (!passes.empty() && (passes.back().tile_mask & 7) == 7), QSLHIP_EINVAL,
+               "run sums need qubits 0..2 in the last tile");
     std::vector<RegOp> rops;
     std::vector<uint32_t> first(passes.size() + 1);
     for (size_t i = 0; i < passes.size(); i++) {
@@ -474,16 +479,17 @@ void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
         const uint32_t n_ops = first[i + 1] - first[i];
         const TilePass &p = passes[i];
         const uint64_t b = i == 0 ? basis : ~0ull;  // only the first pass reads the state
+        double *sm = i + 1 == passes.size() ? runs : nullptr;  // the last one writes sums
         if (cbytes && jit_launch_pass(h, amps, p, threads, gpw, exact, rops.data() + first[i],
-                                      n_ops, dcoef + cfirst[i], b))
+                                      n_ops, dcoef + cfirst[i], b, sm))
             continue;
         const RegOp *d = dops + first[i];
         if (h->prec == 8) {
-            if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw, b);
-            else launch_t<double, false>(h, amps, p, threads, d, n_ops, gpw, b);
+            if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw, b, sm);
+            else launch_t<double, false>(h, amps, p, threads, d, n_ops, gpw, b, sm);
         } else {
-            if (exact) launch_t<float, true>(h, amps, p, threads, d, n_ops, gpw, b);
-            else launch_t<float, false>(h, amps, p, threads, d, n_ops, gpw, b);
+            if (exact) launch_t<float, true>(h, amps, p, threads, d, n_ops, gpw, b, sm);
+            else launch_t<float, false>(h, amps, p, threads, d, n_ops, gpw, b, sm);
         }
     }
 }

@@ -49,10 +49,10 @@ uint64_t reg_grid(uint64_t groups, int gpw);
 // (or every pass, tuning "jit" = 2) runs as straight-line code generated
 // from its decoded ops.  Returns false when the pass should take the
 // ahead-of-time kernel instead (not repeated yet, or the compiler is not
-// available); then nothing was launched.  basis: as RegTileArgs::basis.
+// available); then nothing was launched.  basis, runs: as in RegTileArgs.
 bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads, int gpw,
                      bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef,
-                     uint64_t basis);
+                     uint64_t basis, double *runs);
 // the gate coefficients (c, s) of a pass's ops in the order the generated
 // code reads them
 void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef);

@@ -37,7 +37,28 @@ struct RegTileArgs {
     uint32_t koff0[kRegMaxR];    // swizzled LDS byte offsets of the load layout
     uint64_t basis;              // ~0: load the state; else the state is this
                                  // basis state, synthesized without a read
+    double *runs;                // non-null (tile holds qubits 0..2): write the
+                                 // |a|^2 sum of every 8-amplitude run here
 };
+
+// The sampling sums of the amplitudes about to be stored, in the pass's
+// group order: with qubits 0..2 on lane bits 0..2, eight lanes of register
+// k hold one 8-amplitude run, tile index (t | k << tb) with bits 0..2
+// clear, summed as the sampler sums a run of the stored state (amp_prob,
+// lane_sum_up<3>).  Run l of group g goes to runs[g * 2^(K-3) + l]; the
+// eight runs of a wavefront's register are 64 contiguous bytes.
+template <typename T, int R, int NT>
+__device__ __forceinline__ void reg_run_sums(const cplx<T> (&v)[R], double *runs, uint64_t g,
+                                             uint32_t t)
+{
+    constexpr int tb = NT == 256 ? 8 : 9;
+    constexpr int lb = tb - 3 + (R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : 5);  // K - 3
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const double s = lane_sum_up<3>(amp_prob(v[k]));
+        if ((t & 7u) == 0) runs[(g << lb) + ((t >> 3) | ((uint32_t)k << (tb - 3)))] = s;
+    }
+}
 
 // One thread's R amplitudes of a group: state indices idx0 | kpart[k].  With
 // a basis index the state is |basis>, never written to memory: 1 + 0i at

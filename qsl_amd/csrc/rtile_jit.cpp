@@ -60,7 +60,11 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
       << "constexpr int R = " << R << ", NT = " << NT << ", A = " << (prec == 8 ? 16 : 8)
       << ", r = " << r << ";\n"
       << "constexpr bool EX = " << (exact ? "true" : "false") << ";\n"
-      << "extern \"C\" __global__ __launch_bounds__(NT) void qslhip_pass(\n"
+      << "extern \"C\" __global__ __launch_bounds__(NT) ";
+    if (tuning().jit_waves > 0)  // caps registers at 512 / waves per lane
+        o << "__attribute__((amdgpu_waves_per_eu(" << tuning().jit_waves << ", "
+          << tuning().jit_waves << "))) ";
+    o << "void qslhip_pass(\n"
          "    cplx<T> *__restrict__ st, unsigned n, RegTileArgs ta,\n"
          "    const double *__restrict__ cf, unsigned long long ngroups)\n{\n"
          "    extern __shared__ __align__(16) unsigned char smem[];\n"
@@ -166,7 +170,8 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
         o << "        }\n";
         j++;
     }
-    o << "#pragma unroll\n"
+    o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);\n"
+         "#pragma unroll\n"
          "        for (int k = 0; k < R; k++) st[gbase | tthread | kpart[k]] = v[k];\n"
          "        if (!more) break;\n"
          "        g = gn;\n"
@@ -252,7 +257,7 @@ void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef
 
 bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads, int gpw,
                      bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef,
-                     uint64_t basis)
+                     uint64_t basis, double *runs)
 {
     const int mode = tuning().jit;
     if (mode == 0) return false;
@@ -289,6 +294,7 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     unsigned n = h->n;
     RegTileArgs targ = ta;
     targ.basis = basis;
+    targ.runs = runs;
     const double *cf = dev_coef;
     unsigned long long ng = groups;
     void *args[] = {&st, &n, &targ, &cf, &ng};
