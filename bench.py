@@ -303,18 +303,39 @@ def bench_c5(layers34=6, layers30=30):
     res.update({"nqubits": n, "layers": layers34, "gates": len(ops), "ms": round(ms, 1),
                 "gates_per_s": round(len(ops) / (ms * 1e-3), 1),
                 "norm_err": abs(q.norm() - 1)})
+    # the same circuit on FP32 register-tile passes, reordered: the first
+    # application interprets, the second compiles, the third is timed
+    times = []
+    for _ in range(3):
+        q.reset()
+        q.sync()
+        ev.record(e0, st)
+        q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+        ev.record(e1, st)
+        times.append(ev.elapsed_ms(e0, e1))
+    npass = len(L.plan_order(n, 4, ops, L.MODE_REORDER, cap=1 << 16)[0])
+    res["register_tile_fp32"] = {
+        "mode": "reorder", "passes": npass, "ms": round(times[2], 1),
+        "gates_per_s": round(len(ops) / (times[2] * 1e-3), 1),
+        "interpreted_ms": round(times[0], 1), "norm_err": abs(q.norm() - 1),
+        "hbm_GBps": round(npass * 2 * (8 << n) / (times[2] * 1e-3) / 1e9, 1)}
     del q
     n = 30
     ops, _ = W.c5_haar_layers(n, layers30)
     q32 = qsl_amd.Qubits(n, dtype=np.complex64)
     q32.apply_ops(ops, mode=qsl_amd.MODE_DENSE)
     s32 = q32.getState()
+    q32.reset()
+    q32.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    s32r = q32.getState()
     del q32
     q64 = qsl_amd.Qubits(n)
     q64.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
     s64 = q64.getState()
     del q64
     res["max_amp_err_30q_vs_fp64"] = float(np.abs(s32.astype(np.complex128) - s64).max())
+    res["register_tile_fp32"]["max_amp_err_30q_vs_fp64"] = float(
+        np.abs(s32r.astype(np.complex128) - s64).max())
     res["max_amp_30q"] = float(np.abs(s64).max())
     return res
 

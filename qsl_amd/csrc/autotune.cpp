@@ -114,6 +114,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "jit_waves") *value = t.jit_waves;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
+        else if (k == "run_sums_used") *value = g_run_sums_used;
         else throw Error(QSLHIP_EINVAL, "unknown tuning key " + k);
     });
 }

@@ -148,7 +148,8 @@ double device_norm2(qslhip_state *h);
 void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome,
                      double factor);
 // where a gate pass writes the sampling sums of the state's 8-amplitude
-// runs (null when the state is too small for the sampler's layout)
+// runs (null when the state is too small for the sampler's layout, or the
+// handle has not sampled yet and holds no buffer for them)
 double *sample_run_sums(qslhip_state *h);
 void device_sample(qslhip_state *h, const double *u, uint64_t nshots,
                    uint64_t *out, bool strict = false);
@@ -203,6 +204,7 @@ struct Tuning {
 };
 Tuning &tuning();
 extern int g_jit_compiled, g_jit_failed;  // passes compiled at run time / failed
+extern int g_run_sums_used;  // samplings that took a gate pass's run sums
 
 inline unsigned grid_for(uint64_t items, unsigned per_block)
 {

@@ -20,6 +20,9 @@
 #include "gate_math.h"
 
 namespace qslhip {
+
+int g_run_sums_used = 0;
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -524,6 +527,7 @@ void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *ou
     double *pre = subs - (nchunks + 1);
     const cplx<T> *st = static_cast<const cplx<T> *>(h->amps);
     if (h->sums_valid) {  // the last gate pass wrote the run sums
+        g_run_sums_used++;
         const uint64_t nsubs = nchunks * kSubsPerChunk;
         const RunLayout L = run_layout(h);
         const uint64_t nblocks = 1ull << (h->n - h->sums_logt - 3);
@@ -560,7 +564,10 @@ double *sample_run_sums(qslhip_state *h)
 {
     // whole chunks only: every sub-chunk sum k_chunk_from_subs reads is written
     if (h->np || h->namps % kScanChunk) return nullptr;
-    const uint64_t nsubs = (h->namps / kScanChunk) * kSubsPerChunk;
+    const uint64_t nchunks = h->namps / kScanChunk, nsubs = nchunks * kSubsPerChunk;
+    // only a handle that has sampled before holds the buffer (namps bytes of
+    // run sums); gate passes never allocate it
+    if (h->cdf.bytes < (nchunks + 1 + nsubs + nsubs * 8) * sizeof(double)) return nullptr;
     return run_sums_buffer(h) + nsubs;
 }
 

@@ -58,6 +58,17 @@ def _counts2(q, u):
     return counts
 
 
+def _sampled(q):
+    # a first sampling gives the handle its sums buffer; gate passes only
+    # write run sums into one that exists
+    q.sampleAllOutcomes(np.array([0.5]))
+    return q
+
+
+def _used():
+    return L.tuning_get("run_sums_used")
+
+
 def _drop_sums(q):
     # a state change that leaves the amplitudes as they are
     q.setAmps(q.getAmps(0, 1), 0)
@@ -104,11 +115,13 @@ def test_outcomes_equal_with_and_without_pass_sums(jit_mode, jit, mode, dtype):
     jit_mode(jit)
     rng = np.random.default_rng(60 + jit + 4 * mode)
     ops = _ops(rng, n, 150, dtype)
-    q = qsl_amd.Qubits(n, dtype=dtype)
+    q = _sampled(qsl_amd.Qubits(n, dtype=dtype))
     q.apply_ops(ops, mode=mode)
     u = qsl_amd.uniforms(7 + mode, 30000)
     u[:3] = [0.0, 0.5, 1.0 - 2 ** -53]
+    used = _used()
     a = q.sampleAllOutcomes(u)
+    assert _used() == used + 1  # the pass's sums were taken
     _drop_sums(q)
     b = q.sampleAllOutcomes(u)
     assert np.array_equal(a, b)
@@ -138,9 +151,11 @@ def test_run_layout_of_every_tile_width(widths, cfg, dtype):
     for seed in range(3):
         rng = np.random.default_rng(100 * logt + 10 * low + seed)
         ops = _ops(rng, n, 60 + 40 * seed, dtype)
-        q = qsl_amd.Qubits(n, dtype=dtype)
+        q = _sampled(qsl_amd.Qubits(n, dtype=dtype))
         q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+        used = _used()
         a = q.sampleAllOutcomes(u)
+        assert _used() == used + 1
         _drop_sums(q)
         assert np.array_equal(a, q.sampleAllOutcomes(u)), seed
     _check_oracle(q.getState(), u, a, dtype)
@@ -149,9 +164,10 @@ def test_run_layout_of_every_tile_width(widths, cfg, dtype):
 @pytest.mark.parametrize("change", ["gate1", "gate2", "apply_per_gate", "collapse", "set_state",
                                     "reset", "set_basis", "expval_grad"])
 def test_state_changes_drop_the_sums(change):
+    # (a handle that never sampled gets no sums: see the first test)
     n = 20
     rng = np.random.default_rng(80)
-    q = qsl_amd.Qubits(n)
+    q = _sampled(qsl_amd.Qubits(n))
     q.apply_ops(_ops(rng, n, 120, np.complex128), mode=qsl_amd.MODE_FUSED)
     if change == "gate1":
         q.hadamard(0)
@@ -176,7 +192,9 @@ def test_state_changes_drop_the_sums(change):
         terms[0]["zmask"] = 1
         q.expvalGrad(np.array([(5, 2, 0, 0, 0.4)], OP_DTYPE), terms)
     u = qsl_amd.uniforms(81, 20000)
+    used = _used()
     got = q.sampleAllOutcomes(u)
+    assert _used() == used  # the change dropped them
     _check_oracle(q.getState(), u, got, np.complex128)
 
 
@@ -184,10 +202,30 @@ def test_measure_all_after_gates():
     n = 20
     rng = np.random.default_rng(90)
     ops = _ops(rng, n, 100, np.complex128)
-    q = qsl_amd.Qubits(n)
+    q = _sampled(qsl_amd.Qubits(n))
     q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
     st = q.getState()
+    used = _used()
     o = q.measureAll(0.4321)
+    assert _used() == used + 1
     assert o == O.sample_all(st, np.array([0.4321]))[0]
     s = q.getState()
     assert s[o] == 1 and np.count_nonzero(s) == 1
+
+
+def test_no_sums_before_the_first_sampling():
+    # a handle that never sampled holds no sums buffer, so its gate passes
+    # write none and its first sampling makes its own pass over the state
+    n = 20
+    rng = np.random.default_rng(91)
+    q = qsl_amd.Qubits(n)
+    q.apply_ops(_ops(rng, n, 80, np.complex128), mode=qsl_amd.MODE_REORDER)
+    used = _used()
+    u = qsl_amd.uniforms(92, 5000)
+    got = q.sampleAllOutcomes(u)
+    assert _used() == used
+    _check_oracle(q.getState(), u, got, np.complex128)
+    q.apply_ops(_ops(rng, n, 80, np.complex128), mode=qsl_amd.MODE_REORDER)
+    got = q.sampleAllOutcomes(u)
+    assert _used() == used + 1
+    _check_oracle(q.getState(), u, got, np.complex128)
