@@ -206,6 +206,8 @@ void plan_tile_passes_reorder(unsigned n, const std::vector<GateDesc> &gates, in
     for (size_t i = 0; i < m; i++)
         if (!npred[i]) ready.insert((uint32_t)i);
     size_t remaining = m;
+    std::vector<std::vector<uint32_t>> takes;
+    std::vector<uint64_t> masks;
     while (remaining) {
         uint64_t mask = base;
         std::vector<uint32_t> take;
@@ -235,16 +237,55 @@ void plan_tile_passes_reorder(unsigned n, const std::vector<GateDesc> &gates, in
         QH_REQUIRE(!take.empty(), QSLHIP_EINVAL, "tile too narrow for a gate's targets");
         uint64_t mk = mask;
         for (unsigned q = 0; q < n && __builtin_popcountll(mk) < K; q++) mk |= 1ull << q;
-        TilePass p;
-        p.logt = K;
-        p.tile_mask = mk;
-        p.op_first = (uint32_t)ops.size();
-        for (uint32_t i : take) {
-            ops.push_back(tile_op_of(gates[i], mk));
-            order.push_back(i);
+        takes.push_back(std::move(take));
+        masks.push_back(mk);
+    }
+    // Balance: the sweep puts every ready diagonal gate into the earliest
+    // pass, which makes the first pass the longest.  A diagonal gate needs
+    // no tile qubit, so it may run in any later pass up to the first one
+    // holding one of its successors; walking the gates backwards, each moves
+    // to the lightest such pass while that one is lighter by two or more.
+    // It runs first there: its predecessors are all in earlier passes, and
+    // every op of that pass that it does not commute with is a successor.
+    const size_t np = takes.size();
+    std::vector<uint32_t> pass_of(m), count(np);
+    for (size_t p = 0; p < np; p++) {
+        count[p] = (uint32_t)takes[p].size();
+        for (uint32_t i : takes[p]) pass_of[i] = (uint32_t)p;
+    }
+    std::vector<char> moved(m, 0);
+    for (size_t i = m; i-- > 0;) {
+        if (need[i]) continue;
+        const uint32_t p = pass_of[i];
+        uint32_t lim = (uint32_t)np - 1;
+        for (uint32_t s : succ[i]) lim = pass_of[s] < lim ? pass_of[s] : lim;
+        uint32_t best = p;
+        for (uint32_t q = p + 1; q <= lim && q < np; q++)
+            if (count[q] < count[best]) best = q;
+        if (best != p && count[best] + 1 < count[p]) {
+            pass_of[i] = best;
+            count[p]--;
+            count[best]++;
+            moved[i] = 1;
         }
-        p.op_count = (uint32_t)take.size();
-        passes.push_back(p);
+    }
+    std::vector<std::vector<uint32_t>> front(np);  // moved-in gates, program order
+    for (size_t i = 0; i < m; i++)
+        if (moved[i]) front[pass_of[i]].push_back((uint32_t)i);
+    for (size_t p = 0; p < np; p++) {
+        TilePass tp;
+        tp.logt = K;
+        tp.tile_mask = masks[p];
+        tp.op_first = (uint32_t)ops.size();
+        auto emit = [&](uint32_t i) {
+            ops.push_back(tile_op_of(gates[i], masks[p]));
+            order.push_back(i);
+        };
+        for (uint32_t i : front[p]) emit(i);
+        for (uint32_t i : takes[p])
+            if (pass_of[i] == p) emit(i);
+        tp.op_count = (uint32_t)(ops.size() - tp.op_first);
+        passes.push_back(tp);
     }
 }
 
