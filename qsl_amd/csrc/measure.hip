@@ -152,6 +152,51 @@ __global__ __launch_bounds__(1024) void k_scan_sums(double *__restrict__ sums, u
     if (threadIdx.x == 1023) sums[n] = run;  // the total: the last run's end
 }
 
+// Longer scans, in three steps: each workgroup scans 1024 entries in place
+// (lane runs of 4, then a block scan of the run totals) and leaves its
+// total; k_scan_sums scans the totals; k_scan_add offsets every entry.
+constexpr int kScanPer = 4;
+constexpr int kScanBlockN = kBlock * kScanPer;
+
+__global__ __launch_bounds__(kBlock) void k_scan_local(double *__restrict__ sums, uint64_t n,
+                                                       double *__restrict__ tot)
+{
+    __shared__ double sh[kBlock];
+    const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlockN + threadIdx.x * kScanPer;
+    double v[kScanPer], run = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+        v[j] = b0 + j < n ? sums[b0 + j] : 0.0;
+        run += v[j];
+    }
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const double o = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        sh[threadIdx.x] += o;
+        __syncthreads();
+    }
+    double ex = threadIdx.x ? sh[threadIdx.x - 1] : 0.0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+        if (b0 + j < n) sums[b0 + j] = ex;
+        ex += v[j];
+    }
+    if (threadIdx.x == kBlock - 1) tot[blockIdx.x] = sh[kBlock - 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(double *__restrict__ sums, uint64_t n,
+                                                     const double *__restrict__ tot, uint64_t nb)
+{
+    const uint64_t b0 = (uint64_t)blockIdx.x * kScanBlockN + threadIdx.x * kScanPer;
+    const double off = tot[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++)
+        if (b0 + j < n) sums[b0 + j] += off;
+    if (blockIdx.x == 0 && threadIdx.x == 0) sums[n] = tot[nb];  // the total
+}
+
 // Scan pass 3: inclusive CDF inside each chunk, offset by the chunk prefix.
 // Lane l owns amplitudes [l*kScanItems, (l+1)*kScanItems) of the chunk so
 // that its serial sum follows index order; loads go through LDS to stay
@@ -428,6 +473,23 @@ double prob_impl(qslhip_state *h, int t, unsigned outcome)
     return v;
 }
 
+// Exclusive scan of sums[0..n) in place, with the total in sums[n].
+void scan_exclusive(qslhip_state *h, double *sums, uint64_t n)
+{
+    if (n <= 4096) {
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, sums, n);
+        return;
+    }
+    const uint64_t nb = (n + kScanBlockN - 1) / kScanBlockN;
+    h->red.ensure((nb + 1) * sizeof(double));  // block totals; free between reductions
+    double *tot = static_cast<double *>(h->red.ptr);
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(kBlock), 0, h->stream, sums, n,
+                       tot);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, tot, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kBlock), 0, h->stream, sums, n,
+                       static_cast<const double *>(tot), nb);
+}
+
 template <typename T>
 void build_cdf(qslhip_state *h)
 {
@@ -439,7 +501,7 @@ void build_cdf(qslhip_state *h)
     const cplx<T> *st = static_cast<const cplx<T> *>(h->amps);
     hipLaunchKernelGGL(k_chunk_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0,
                        h->stream, st, namps, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, sums, nchunks);
+    scan_exclusive(h, sums, nchunks);
     hipLaunchKernelGGL(k_chunk_scan<T>, dim3((unsigned)nchunks), dim3(kBlock), 0,
                        h->stream, st, namps, sums, cdf);
     QH_CHECK(hipGetLastError());
@@ -539,7 +601,7 @@ void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *ou
         hipLaunchKernelGGL(k_sub_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0, h->stream,
                            st, namps, subs, pre);
     }
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, pre, nchunks);
+    scan_exclusive(h, pre, nchunks);
     QH_CHECK(hipGetLastError());
     if (nshots == 0) {
         QH_CHECK(hipStreamSynchronize(h->stream));

@@ -229,3 +229,20 @@ def test_no_sums_before_the_first_sampling():
     got = q.sampleAllOutcomes(u)
     assert _used() == used + 1
     _check_oracle(q.getState(), u, got, np.complex128)
+
+
+def test_sampling_at_26_qubits():
+    # 2^14 chunk prefixes: the multi-workgroup scan (k_scan_local, k_scan_add)
+    n = 26
+    rng = np.random.default_rng(26)
+    ops = _ops(rng, n, 120, np.complex128)
+    q = _sampled(qsl_amd.Qubits(n))
+    q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    u = qsl_amd.uniforms(2626, 20000)
+    u[:2] = [0.0, 1.0 - 2 ** -53]
+    used = _used()
+    a = q.sampleAllOutcomes(u)
+    assert _used() == used + 1
+    _drop_sums(q)
+    assert np.array_equal(a, q.sampleAllOutcomes(u))
+    _check_oracle(q.getState(), u, a, np.complex128)
