@@ -135,8 +135,8 @@ bool launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
         // and eight groups for the sampler to regroup them (k_subs_from_runs)
         if (runs && ((passes.back().tile_mask & 7) != 7 || h->n < (unsigned)passes.back().logt + 3))
             runs = nullptr;
-        launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups, exact, basis, runs);
-        return runs != nullptr;
+        return launch_reg_passes(h, passes, ops, amps, threads, tu.rgroups, exact, basis, runs) &&
+               runs != nullptr;
     }
     // the other kernels read the state: write the basis state first
     if (basis != qslhip_state::kNoBasis) launch_fill_basis(h, basis);

@@ -137,7 +137,8 @@ bool launch_tile_passes(qslhip_state *h, const std::vector<TilePass> &passes,
 // the same passes with the amplitudes held in registers, `threads` (256 or
 // 512) per workgroup (regtile.hip); reg_tile_ok says whether a kernel
 // exists for the tile width
-void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
+// Returns whether the last pass wrote the run sums.
+bool launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                        const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
                        bool exact, uint64_t basis = qslhip_state::kNoBasis,
                        double *runs = nullptr);

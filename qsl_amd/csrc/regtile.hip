@@ -62,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
     }
     uint64_t g = blockIdx.x;
     if (g >= ngroups) return;
-    uint64_t gbase_n = deposit(g, outside);
+    uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;
     cplx<T> v[R], vn[R];
     reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);
     wait_vmcnt<0>();
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
         const uint64_t gn = g + gridDim.x;
         const bool more = gn < ngroups;
         if (more) {
-            gbase_n = deposit(gn, outside);
+            gbase_n = deposit(gn, ta.gvar) | ta.gfix;
             reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);
             // in flight: the previous group's R stores and these R loads;
             // everything older (this group's loads) is complete
@@ -86,7 +86,11 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
         uint32_t koff[R];
 #pragma unroll
         for (int k = 0; k < R; k++) koff[k] = ta.koff0[k];
-        for (int i = 0; i < nops; i++) {
+        // from a basis state, a group without the basis index is zero and
+        // stays zero (gates never mix groups): store it as it is.  Program
+        // order (EX) runs the ops anyway: they may turn +0 into -0 there
+        const int gops = (!EX && ta.basis != ~0ull && (ta.basis & outside) != gbase) ? 0 : nops;
+        for (int i = 0; i < gops; i++) {
             const RegOpHead &o = ops[i].h;
             const int code = o.code;
             if (code == RC_REMAP) {
@@ -354,6 +358,8 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
     ta.basis = ~0ull;
     ta.runs = nullptr;
     ta.tile_mask = p.tile_mask;
+    ta.gvar = ((n >= 64) ? ~0ull : ((1ull << n) - 1)) & ~p.tile_mask;
+    ta.gfix = 0;
     int pos = 0;
     for (unsigned q = 0; q < 64; q++) {
         if (!((p.tile_mask >> q) & 1ull)) continue;
@@ -370,6 +376,23 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
     return ta;
 }
 
+RegTileArgs pass_args(unsigned n, const TilePass &p, int tb, int A, const PassExtras &x,
+                      uint64_t *groups)
+{
+    RegTileArgs ta = make_tile_args(n, p, tb, A);
+    ta.basis = x.basis;
+    ta.runs = x.runs;
+    if (x.gvar != ~0ull) {
+        QH_REQUIRE((x.gvar & ~ta.gvar) == 0 && (x.gfix & ~ta.gvar) == 0 && (x.gvar & x.gfix) == 0,
+                   QSLHIP_EINVAL, "group subset outside the pass's outside qubits");
+        QH_REQUIRE(!x.runs || x.gvar == ta.gvar, QSLHIP_EINVAL, "run sums need every group");
+        ta.gvar = x.gvar;
+        ta.gfix = x.gfix;
+    }
+    *groups = 1ull << __builtin_popcountll(ta.gvar);
+    return ta;
+}
+
 uint64_t reg_grid(uint64_t groups, int gpw)
 {
     // several groups per workgroup only when groups are plentiful
@@ -381,17 +404,15 @@ namespace {
 
 template <typename T, int NT, int R, bool EX>
 void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dops, uint32_t nops,
-                int gpw, uint64_t basis, double *runs)
+                int gpw, const PassExtras &x)
 {
     constexpr int A = (int)sizeof(cplx<T>);
     constexpr int tb = NT == 256 ? 8 : 9;
     constexpr int r = R == 4 ? 2 : R == 8 ? 3 : 4;
     const int K = tb + r;
-    RegTileArgs ta = make_tile_args(h->n, p, tb, A);
-    ta.basis = basis;
-    ta.runs = runs;
+    uint64_t groups = 0;
+    const RegTileArgs ta = pass_args(h->n, p, tb, A, x, &groups);
     const size_t shmem = (size_t)A << K;
-    const uint64_t groups = 1ull << (h->n - K);
     uint64_t grid;
     if (gpw > 0) {
         grid = reg_grid(groups, gpw);
@@ -414,14 +435,14 @@ void launch_one(qslhip_state *h, void *amps, const TilePass &p, const RegOp *dop
 
 template <typename T, bool EX>
 void launch_t(qslhip_state *h, void *amps, const TilePass &p, int nt, const RegOp *dops,
-              uint32_t nops, int gpw, uint64_t basis, double *runs)
+              uint32_t nops, int gpw, const PassExtras &x)
 {
     const int r = p.logt - (nt == 512 ? 9 : 8);
-    if (nt == 256 && r == 2) launch_one<T, 256, 4, EX>(h, amps, p, dops, nops, gpw, basis, runs);
-    else if (nt == 256 && r == 3) launch_one<T, 256, 8, EX>(h, amps, p, dops, nops, gpw, basis, runs);
-    else if (nt == 256 && r == 4) launch_one<T, 256, 16, EX>(h, amps, p, dops, nops, gpw, basis, runs);
-    else if (nt == 512 && r == 3) launch_one<T, 512, 8, EX>(h, amps, p, dops, nops, gpw, basis, runs);
-    else if (nt == 512 && r == 4) launch_one<T, 512, 16, EX>(h, amps, p, dops, nops, gpw, basis, runs);
+    if (nt == 256 && r == 2) launch_one<T, 256, 4, EX>(h, amps, p, dops, nops, gpw, x);
+    else if (nt == 256 && r == 3) launch_one<T, 256, 8, EX>(h, amps, p, dops, nops, gpw, x);
+    else if (nt == 256 && r == 4) launch_one<T, 256, 16, EX>(h, amps, p, dops, nops, gpw, x);
+    else if (nt == 512 && r == 3) launch_one<T, 512, 8, EX>(h, amps, p, dops, nops, gpw, x);
+    else if (nt == 512 && r == 4) launch_one<T, 512, 16, EX>(h, amps, p, dops, nops, gpw, x);
     else throw Error(QSLHIP_EINVAL, "no register-tile kernel for this tile width");
 }
 
@@ -435,7 +456,35 @@ bool reg_tile_ok(int logt, int threads, int prec)
     return ((size_t)(prec == 8 ? 16 : 8) << logt) <= 64 * 1024;
 }
 
-void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
+static std::vector<PassExtras> pass_extras(unsigned n, const std::vector<TilePass> &passes,
+                                           bool exact, uint64_t basis, double *runs)
+{
+    std::vector<PassExtras> xs(passes.size());
+    const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+    uint64_t support = 0;  // qubits a basis state's amplitudes may vary in so far
+    for (size_t i = 0; i < passes.size(); i++) {
+        PassExtras &x = xs[i];
+        const uint64_t outside = full & ~passes[i].tile_mask;
+        if (i == 0) x.basis = basis;  // only the first pass reads the state
+        if (basis != ~0ull && !exact && i > 0) {
+            // Reordered mode from a basis state: the amplitudes are zero
+            // outside the groups that agree with the basis on every qubit
+            // no earlier tile held (the first pass wrote those zeros, and
+            // gates never mix groups), so only those groups run.  Program
+            // order runs every group: gates on zeros may leave -0 there.
+            x.gvar = outside & support;
+            x.gfix = basis & outside & ~support;
+        }
+        support |= passes[i].tile_mask;
+    }
+    // the last pass writes the run sums when it covers every group
+    if (!xs.empty() && (xs.back().gvar == ~0ull ||
+                        xs.back().gvar == (full & ~passes.back().tile_mask)))
+        xs.back().runs = runs;
+    return xs;
+}
+
+bool launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                        const std::vector<TileOp> &ops, void *amps, int threads, int gpw,
                        bool exact, uint64_t basis, double *runs)
 {
@@ -475,23 +524,24 @@ void launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
     const RegOp *dops = static_cast<const RegOp *>(h->optab.ptr);
     const double *dcoef =
         reinterpret_cast<const double *>(static_cast<char *>(h->optab.ptr) + bytes);
+    const std::vector<PassExtras> xs = pass_extras(h->n, passes, exact, basis, runs);
     for (size_t i = 0; i < passes.size(); i++) {
         const uint32_t n_ops = first[i + 1] - first[i];
         const TilePass &p = passes[i];
-        const uint64_t b = i == 0 ? basis : ~0ull;  // only the first pass reads the state
-        double *sm = i + 1 == passes.size() ? runs : nullptr;  // the last one writes sums
+        const PassExtras &x = xs[i];
         if (cbytes && jit_launch_pass(h, amps, p, threads, gpw, exact, rops.data() + first[i],
-                                      n_ops, dcoef + cfirst[i], b, sm))
+                                      n_ops, dcoef + cfirst[i], x))
             continue;
         const RegOp *d = dops + first[i];
         if (h->prec == 8) {
-            if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw, b, sm);
-            else launch_t<double, false>(h, amps, p, threads, d, n_ops, gpw, b, sm);
+            if (exact) launch_t<double, true>(h, amps, p, threads, d, n_ops, gpw, x);
+            else launch_t<double, false>(h, amps, p, threads, d, n_ops, gpw, x);
         } else {
-            if (exact) launch_t<float, true>(h, amps, p, threads, d, n_ops, gpw, b, sm);
-            else launch_t<float, false>(h, amps, p, threads, d, n_ops, gpw, b, sm);
+            if (exact) launch_t<float, true>(h, amps, p, threads, d, n_ops, gpw, x);
+            else launch_t<float, false>(h, amps, p, threads, d, n_ops, gpw, x);
         }
     }
+    return xs.empty() || xs.back().runs != nullptr;
 }
 
 }  // namespace qslhip

@@ -45,14 +45,26 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A);
 // grid of a pass: gpw groups per workgroup when groups are plentiful
 uint64_t reg_grid(uint64_t groups, int gpw);
 
+// Per-launch settings of a pass beyond its ops (defaults: load the state,
+// no run sums, every group).
+struct PassExtras {
+    uint64_t basis = ~0ull;  // RegTileArgs::basis
+    double *runs = nullptr;  // RegTileArgs::runs
+    uint64_t gvar = ~0ull;   // ~0: every group; else RegTileArgs::gvar / gfix
+    uint64_t gfix = 0;
+};
+// the tile arguments of a pass with its extras applied, and its group count
+RegTileArgs pass_args(unsigned n, const TilePass &p, int tb, int A, const PassExtras &x,
+                      uint64_t *groups);
+
 // Passes compiled at run time (rtile_jit.cpp): a pass structure seen before
 // (or every pass, tuning "jit" = 2) runs as straight-line code generated
 // from its decoded ops.  Returns false when the pass should take the
 // ahead-of-time kernel instead (not repeated yet, or the compiler is not
-// available); then nothing was launched.  basis, runs: as in RegTileArgs.
+// available); then nothing was launched.
 bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads, int gpw,
                      bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef,
-                     uint64_t basis, double *runs);
+                     const PassExtras &x);
 // the gate coefficients (c, s) of a pass's ops in the order the generated
 // code reads them
 void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef);

@@ -39,6 +39,9 @@ struct RegTileArgs {
                                  // basis state, synthesized without a read
     double *runs;                // non-null (tile holds qubits 0..2): write the
                                  // |a|^2 sum of every 8-amplitude run here
+    uint64_t gvar, gfix;         // group g starts at deposit(g, gvar) | gfix:
+                                 // gvar = the outside qubits, gfix = 0 for a
+                                 // pass over every group
 };
 
 // The sampling sums of the amplitudes about to be stored, in the pass's

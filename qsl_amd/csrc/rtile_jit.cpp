@@ -83,7 +83,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "    }\n"
          "    uint64_t g = blockIdx.x;\n"
          "    if (g >= ngroups) return;\n"
-         "    uint64_t gbase_n = deposit(g, outside);\n"
+         "    uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;\n"
          "    cplx<T> v[R], vn[R];\n"
          "    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);\n"
          "    wait_vmcnt<0>();\n"
@@ -94,7 +94,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "        const uint64_t gn = g + gridDim.x;\n"
          "        const bool more = gn < ngroups;\n"
          "        if (more) {\n"
-         "            gbase_n = deposit(gn, outside);\n"
+         "            gbase_n = deposit(gn, ta.gvar) | ta.gfix;\n"
          "            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);\n"
          "            wait_vmcnt<2 * R>();\n"
          "        } else {\n"
@@ -102,7 +102,9 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "        }\n"
          "        uint32_t jt = t;\n"
          "        uint32_t base = lds_swz<A>(jt) * A;\n"
-         "        (void)gbase; (void)jt; (void)base;\n";
+         "        (void)gbase; (void)jt; (void)base;\n"
+         "        // a zero group of a basis state stays zero (reordered mode)\n"
+         "        if (EX || ta.basis == ~0ull || (ta.basis & outside) == gbase) {\n";
     std::vector<uint32_t> kcur(koff0, koff0 + R);
     int j = 0;  // coefficient pair index
     for (uint32_t i = 0; i < nops; i++) {
@@ -170,7 +172,8 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
         o << "        }\n";
         j++;
     }
-    o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);\n"
+    o << "        }\n"
+         "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);\n"
          "#pragma unroll\n"
          "        for (int k = 0; k < R; k++) st[gbase | tthread | kpart[k]] = v[k];\n"
          "        if (!more) break;\n"
@@ -257,7 +260,7 @@ void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef
 
 bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads, int gpw,
                      bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef,
-                     uint64_t basis, double *runs)
+                     const PassExtras &x)
 {
     const int mode = tuning().jit;
     if (mode == 0) return false;
@@ -287,14 +290,12 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
         if (it->second.failed) return false;
         fn = it->second.fn;
     }
-    const uint64_t groups = 1ull << (h->n - p.logt);
+    uint64_t groups = 0;
+    RegTileArgs targ = pass_args(h->n, p, tb, A, x, &groups);
     const unsigned grid = (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1);
     const size_t shmem = (size_t)A << p.logt;
     void *st = amps;
     unsigned n = h->n;
-    RegTileArgs targ = ta;
-    targ.basis = basis;
-    targ.runs = runs;
     const double *cf = dev_coef;
     unsigned long long ng = groups;
     void *args[] = {&st, &n, &targ, &cf, &ng};
