@@ -367,9 +367,9 @@ def main():
     ops, stream = W.c2_random(N, NGATES)
     if args.profile:
         # a short fixed run for rocprofv3 counter passes: one sampling, the
-        # circuit twice reordered and twice in program order (the second
-        # application of a pass structure runs its compiled pass), rotateX on
-        # a low and a high qubit, and one
+        # circuit three times reordered and three times in program order (on
+        # the dense state the sampling wrote), rotateX on a low and a high
+        # qubit, and one
         # device-to-device copy of the 16 GiB state (2^30*16 B read and
         # written: the calibration for FETCH_SIZE / WRITE_SIZE)
         q = qsl_amd.Qubits(N, dtype=np.complex128)
@@ -377,11 +377,13 @@ def main():
         # the last pass of each application writes its run sums, as in the
         # timed steps
         q.sampleAllOutcomes(np.array([0.5]))
-        for rep in (1, 2):  # the second application runs the compiled passes
+        # the first application interprets, the second compiles each pass
+        # before launching it, the third runs the compiled passes back to back
+        for rep in (1, 2, 3):
             print(f"profile: reordered circuit ({rep})", flush=True)
             q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
             q.sync()
-        for rep in (1, 2):
+        for rep in (1, 2, 3):
             print(f"profile: program-order circuit ({rep})", flush=True)
             q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
             q.sync()
