@@ -64,7 +64,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
     if (g >= ngroups) return;
     uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;
     cplx<T> v[R], vn[R];
-    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);
+    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);
     wait_vmcnt<0>();
     for (;;) {
         const uint64_t gbase = gbase_n;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
         const bool more = gn < ngroups;
         if (more) {
             gbase_n = deposit(gn, ta.gvar) | ta.gfix;
-            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);
+            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);
             // in flight: the previous group's R stores and these R loads;
             // everything older (this group's loads) is complete
             wait_vmcnt<2 * R>();
@@ -360,6 +360,8 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
     ta.tile_mask = p.tile_mask;
     ta.gvar = ((n >= 64) ? ~0ull : ((1ull << n) - 1)) & ~p.tile_mask;
     ta.gfix = 0;
+    ta.zmask = 0;
+    ta.zval = 0;
     int pos = 0;
     for (unsigned q = 0; q < 64; q++) {
         if (!((p.tile_mask >> q) & 1ull)) continue;
@@ -382,6 +384,10 @@ RegTileArgs pass_args(unsigned n, const TilePass &p, int tb, int A, const PassEx
     RegTileArgs ta = make_tile_args(n, p, tb, A);
     ta.basis = x.basis;
     ta.runs = x.runs;
+    QH_REQUIRE((x.zmask & ~p.tile_mask) == 0 && (x.zval & ~x.zmask) == 0, QSLHIP_EINVAL,
+               "zero mask outside the tile");
+    ta.zmask = x.zmask;
+    ta.zval = x.zval;
     if (x.gvar != ~0ull) {
         QH_REQUIRE((x.gvar & ~ta.gvar) == 0 && (x.gfix & ~ta.gvar) == 0 && (x.gvar & x.gfix) == 0,
                    QSLHIP_EINVAL, "group subset outside the pass's outside qubits");
@@ -461,21 +467,36 @@ static std::vector<PassExtras> pass_extras(unsigned n, const std::vector<TilePas
 {
     std::vector<PassExtras> xs(passes.size());
     const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1);
-    uint64_t support = 0;  // qubits a basis state's amplitudes may vary in so far
-    for (size_t i = 0; i < passes.size(); i++) {
-        PassExtras &x = xs[i];
-        const uint64_t outside = full & ~passes[i].tile_mask;
-        if (i == 0) x.basis = basis;  // only the first pass reads the state
-        if (basis != ~0ull && !exact && i > 0) {
-            // Reordered mode from a basis state: the amplitudes are zero
-            // outside the groups that agree with the basis on every qubit
-            // no earlier tile held (the first pass wrote those zeros, and
-            // gates never mix groups), so only those groups run.  Program
-            // order runs every group: gates on zeros may leave -0 there.
-            x.gvar = outside & support;
-            x.gfix = basis & outside & ~support;
+    if (!xs.empty()) xs[0].basis = basis;  // only the first pass reads the state
+    if (basis != ~0ull && !exact) {
+        // Reordered mode from a basis state.  After passes 0..p-1 the state
+        // can be nonzero only where it agrees with the basis on every qubit
+        // no tile has held yet (gates never mix groups), so pass p runs only
+        // the groups that agree there.  When every qubit is in some tile,
+        // nothing is stored before it is computed: the first pass runs the
+        // basis index's group alone, and a later pass reads only the
+        // amplitudes an earlier one wrote (those agreeing with the basis on
+        // its tile qubits no earlier tile held) and takes the rest as +0.
+        // Otherwise the first pass stores every group, the zero ones
+        // without their ops, and nothing is left unwritten.  Program order
+        // runs every group: gates applied to zeros may leave -0 there.
+        uint64_t all = 0;
+        for (const TilePass &p : passes) all |= p.tile_mask;
+        const bool fill = (all & full) != full;
+        uint64_t support = 0;  // qubits the amplitudes may vary in so far
+        for (size_t i = 0; i < passes.size(); i++) {
+            PassExtras &x = xs[i];
+            const uint64_t tile = passes[i].tile_mask, outside = full & ~tile;
+            if (i > 0 || !fill) {
+                x.gvar = outside & support;
+                x.gfix = basis & outside & ~support;
+            }
+            if (i > 0 && !fill) {
+                x.zmask = tile & ~support;
+                x.zval = basis & x.zmask;
+            }
+            support |= tile;
         }
-        support |= passes[i].tile_mask;
     }
     // the last pass writes the run sums when it covers every group
     if (!xs.empty() && (xs.back().gvar == ~0ull ||

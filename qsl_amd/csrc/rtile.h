@@ -52,6 +52,8 @@ struct PassExtras {
     double *runs = nullptr;  // RegTileArgs::runs
     uint64_t gvar = ~0ull;   // ~0: every group; else RegTileArgs::gvar / gfix
     uint64_t gfix = 0;
+    uint64_t zmask = 0;      // RegTileArgs::zmask / zval
+    uint64_t zval = 0;
 };
 // the tile arguments of a pass with its extras applied, and its group count
 RegTileArgs pass_args(unsigned n, const TilePass &p, int tb, int A, const PassExtras &x,

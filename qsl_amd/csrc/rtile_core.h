@@ -42,6 +42,8 @@ struct RegTileArgs {
     uint64_t gvar, gfix;         // group g starts at deposit(g, gvar) | gfix:
                                  // gvar = the outside qubits, gfix = 0 for a
                                  // pass over every group
+    uint64_t zmask, zval;        // amplitudes whose bits under zmask differ
+                                 // from zval are zero and not in memory yet
 };
 
 // The sampling sums of the amplitudes about to be stored, in the pass's
@@ -64,21 +66,35 @@ __device__ __forceinline__ void reg_run_sums(const cplx<T> (&v)[R], double *runs
 }
 
 // One thread's R amplitudes of a group: state indices idx0 | kpart[k].  With
-// a basis index the state is |basis>, never written to memory: 1 + 0i at
-// that index and +0 elsewhere, the values a fill of the vector would hold.
+// a basis index (ta.basis != ~0) the state is |basis>, never written to
+// memory: 1 + 0i at that index and +0 elsewhere, the values a fill of the
+// vector would hold.  Otherwise an amplitude whose bits under ta.zmask
+// differ from ta.zval is known to be zero and never written yet: it is +0
+// and not loaded (zmask = 0: load everything).
 template <typename T, int R>
 __device__ __forceinline__ void reg_load(cplx<T> (&v)[R], const cplx<T> *st, uint64_t idx0,
-                                         const uint64_t (&kpart)[R], uint64_t basis)
+                                         const uint64_t (&kpart)[R], const RegTileArgs &ta)
 {
-    if (basis == ~0ull) {
-#pragma unroll
-        for (int k = 0; k < R; k++) v[k] = st[idx0 | kpart[k]];
-    } else {
+    if (ta.basis != ~0ull) {
 #pragma unroll
         for (int k = 0; k < R; k++) {
-            v[k].re = (idx0 | kpart[k]) == basis ? T(1) : T(0);
+            v[k].re = (idx0 | kpart[k]) == ta.basis ? T(1) : T(0);
             v[k].im = T(0);
         }
+    } else if (ta.zmask) {
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const uint64_t i = idx0 | kpart[k];
+            if ((i & ta.zmask) == ta.zval) {
+                v[k] = st[i];
+            } else {
+                v[k].re = T(0);
+                v[k].im = T(0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = st[idx0 | kpart[k]];
     }
 }
 

@@ -85,7 +85,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "    if (g >= ngroups) return;\n"
          "    uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;\n"
          "    cplx<T> v[R], vn[R];\n"
-         "    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);\n"
+         "    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
          "    wait_vmcnt<0>();\n"
          "    for (;;) {\n"
          "        const uint64_t gbase = gbase_n;\n"
@@ -95,7 +95,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "        const bool more = gn < ngroups;\n"
          "        if (more) {\n"
          "            gbase_n = deposit(gn, ta.gvar) | ta.gfix;\n"
-         "            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta.basis);\n"
+         "            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
          "            wait_vmcnt<2 * R>();\n"
          "        } else {\n"
          "            wait_vmcnt<R>();\n"

@@ -181,3 +181,29 @@ def test_reordered_passes_over_the_nonzero_groups_only(jit_mode, jit, dtype):
         q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
         O.apply_ops(want, ops)
         assert np.abs(q.getState() - want).max() <= 2 * tol, idx
+
+
+@pytest.mark.parametrize("jit", [0, 2])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_reordered_from_a_basis_state_on_a_used_vector(jit_mode, jit, dtype):
+    # The vector still holds the previous state.  When every qubit is in some
+    # tile the passes store nothing before computing it, reading only what an
+    # earlier pass wrote; otherwise the first pass stores every group.  Both
+    # must leave no trace of the old contents.
+    jit_mode(jit)
+    n = 20
+    rng = np.random.default_rng(3300 + jit)
+    tol = 1e-12 if dtype == np.complex128 else 1e-5
+    every = _ops(rng, n, 150, dtype)             # touches every qubit
+    some = _ops(rng, 12, 80, dtype)              # qubits 0..11 only: 8 never in a tile
+    for ops in (every, some):
+        q = qsl_amd.Qubits(n, dtype=dtype)
+        q.apply_ops(_ops(rng, n, 60, dtype), mode=qsl_amd.MODE_FUSED)  # old contents
+        for idx in (0, int(rng.integers(1, 1 << n))):
+            q.setBasisState(idx)
+            q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+            want = _basis(n, idx, dtype)
+            O.apply_ops(want, ops)
+            got = q.getState()
+            assert np.abs(got - want).max() <= tol, (len(ops), idx)
+            assert abs(np.vdot(got, got).real - 1) <= 1e3 * tol
