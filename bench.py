@@ -564,8 +564,8 @@ def main():
             "mode": args.mode,
             "gate_ms": round(gate_ms, 3),
             "gates_per_s": round(NGATES / (gate_ms * 1e-3), 1),
-            "from_reset": "the first pass synthesises |0..0> and stores its zero groups "
-                          "without their ops; later passes skip the groups still zero",
+            "from_reset": "from |0..0> each pass runs only the groups that can be nonzero "
+                          "and reads only what an earlier pass wrote",
             "steady_state_ms": round(steady_ms, 3),
             "dense_gates_per_s": round(NGATES / (steady_ms * 1e-3), 1),
             "algorithmic_GBps": round(W.algorithmic_bytes(ops, N, 16) / (gate_ms * 1e-3) / 1e9, 1),
