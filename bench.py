@@ -314,11 +314,19 @@ def bench_c5(layers34=6, layers30=30):
         ev.record(e1, st)
         times.append(ev.elapsed_ms(e0, e1))
     npass = len(L.plan_order(n, 4, ops, L.MODE_REORDER, cap=1 << 16)[0])
+    norm_err = abs(q.norm() - 1)
+    ev.record(e0, st)
+    q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)  # on the dense result: every group
+    ev.record(e1, st)
+    dense_ms = ev.elapsed_ms(e0, e1)
     res["register_tile_fp32"] = {
         "mode": "reorder", "passes": npass, "ms": round(times[2], 1),
         "gates_per_s": round(len(ops) / (times[2] * 1e-3), 1),
-        "interpreted_ms": round(times[0], 1), "norm_err": abs(q.norm() - 1),
-        "hbm_GBps": round(npass * 2 * (8 << n) / (times[2] * 1e-3) / 1e9, 1)}
+        "interpreted_ms": round(times[0], 1), "norm_err": norm_err,
+        "from_reset": "the passes skip what is still zero",
+        "dense_ms": round(dense_ms, 1),
+        "dense_gates_per_s": round(len(ops) / (dense_ms * 1e-3), 1),
+        "dense_hbm_GBps": round(npass * 2 * (8 << n) / (dense_ms * 1e-3) / 1e9, 1)}
     del q
     n = 30
     ops, _ = W.c5_haar_layers(n, layers30)
