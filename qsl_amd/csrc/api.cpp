@@ -627,6 +627,10 @@ int qslhip_apply_ops(qslhip_t h, const qslhip_op *ops, uint64_t nops, int mode)
             launch_dense_passes(h, tuning().dense_f, dp, dops, mats);
             return;
         }
+        if (h->np && mode != QSLHIP_MODE_PER_GATE) {
+            np_apply_gates(h, gates);
+            return;
+        }
         if (mode == QSLHIP_MODE_PER_GATE || h->np) {
             materialize(h);
             for (const GateDesc &g : gates) {

@@ -159,6 +159,9 @@ void device_sample(qslhip_state *h, const double *u, uint64_t nshots,
 void np_init(qslhip_state *h);
 void np_free(qslhip_state *h);
 void np_launch_gate(qslhip_state *h, const GateDesc &g);
+// a gate list; runs of diagonal gates go through one pass each, with
+// results bit-identical to np_launch_gate per gate
+void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates);
 double np_prob(qslhip_state *h, unsigned t, unsigned outcome);
 void np_collapse(qslhip_state *h, unsigned t, unsigned outcome, double factor);
 void np_sample(qslhip_state *h, const double *u, uint64_t nshots,

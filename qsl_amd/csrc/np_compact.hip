@@ -178,6 +178,53 @@ __global__ __launch_bounds__(kBlock) void k_np_gate(cplx<T> *__restrict__ amp, N
     }
 }
 
+// One diagonal gate of a fused run (k_np_diag).
+struct NPDiagOp {
+    int32_t cls, body;  // PC_ONE, PC_PAIR1, PC_DIAG2 or PC_CTRL; gate body
+    uint32_t q0, q1;
+    double c, s;        // coefficients, already in the handle's precision
+};
+
+// A run of diagonal gates in one pass over the compact vector: each
+// amplitude is loaded once and takes the run's factors in program order,
+// each with k_np_gate's arithmetic for its class, so the result is
+// bit-identical to one launch per gate.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_diag(cplx<T> *__restrict__ amp, NPDev d,
+                                                    const NPDiagOp *__restrict__ ops, int nops)
+{
+    const Seg g = seg_of(d);
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+#pragma unroll
+    for (int e = 0; e < kItems; e++) {
+        uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        if (i >= g.size || i >= g.i0 + kChunk) continue;
+        const uint64_t x = hix | d.lo_list[g.cstart + i];
+        const uint64_t r = g.off + i;
+        cplx<T> a = amp[r];
+        for (int j = 0; j < nops; j++) {
+            const NPDiagOp o = ops[j];
+            const bool x0 = (x >> o.q0) & 1ull, x1 = (x >> o.q1) & 1ull;
+            const T c = (T)o.c, s = (T)o.s;
+            switch (o.cls) {
+            case PC_ONE:
+                if (x0) one_update<T>(o.body, a, c, s);
+                break;
+            case PC_PAIR1:
+                rz_half<T>(x0, a, c, s);
+                break;
+            case PC_DIAG2:
+                if (x0 && x1) one_update<T>(o.body, a, c, s);
+                break;
+            default:  // PC_CTRL
+                if (x0) rz_half<T>(x1, a, c, s);
+                break;
+            }
+        }
+        amp[r] = a;
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -477,6 +524,81 @@ void np_launch_gate(qslhip_state *h, const GateDesc &g)
 {
     if (h->prec == 8) gate_impl<double>(h, g);
     else gate_impl<float>(h, g);
+}
+
+// The class of an elementwise (diagonal) NP gate, or -1.
+static int diag_class(int kind)
+{
+    switch (kind) {
+    case QSLHIP_PAULI_Z: case QSLHIP_PHASE: return PC_ONE;
+    case QSLHIP_ROT_Z: return PC_PAIR1;
+    case QSLHIP_CZ: case QSLHIP_CPHASE: return PC_DIAG2;
+    case QSLHIP_CROT_Z: return PC_CTRL;
+    default: return -1;
+    }
+}
+
+void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
+{
+    // runs of two or more consecutive diagonal gates become one pass each
+    // (k_np_diag); the ops of all runs are staged in one copy
+    std::vector<NPDiagOp> dops;
+    std::vector<std::pair<size_t, size_t>> runs;  // [first gate, end) of each run
+    for (size_t i = 0; i < gates.size();) {
+        size_t j = i;
+        while (j < gates.size() && diag_class(gates[j].kind) >= 0) j++;
+        if (j - i >= 2) {
+            runs.emplace_back(i, j);
+            for (size_t k = i; k < j; k++) {
+                const GateDesc &g = gates[k];
+                NPDiagOp o;
+                o.cls = diag_class(g.kind);
+                o.body = body_of(g.kind);
+                o.q0 = g.q0;
+                o.q1 = is_gate2(g.kind) ? g.q1 : g.q0;
+                o.c = g.c;
+                o.s = g.s;
+                dops.push_back(o);
+            }
+            i = j;
+        } else {
+            i = j > i ? j : i + 1;
+        }
+    }
+    const NPDiagOp *dev = nullptr;
+    if (!dops.empty()) {
+        const size_t bytes = dops.size() * sizeof(NPDiagOp);
+        if (h->staged) QH_CHECK(hipEventSynchronize(h->staged));  // pin may still be read
+        h->pin.ensure(bytes);
+        h->optab.ensure(bytes);
+        std::memcpy(h->pin.ptr, dops.data(), bytes);
+        QH_CHECK(hipMemcpyAsync(h->optab.ptr, h->pin.ptr, bytes, hipMemcpyHostToDevice,
+                                h->stream));
+        QH_CHECK(hipEventRecord(h->staged, h->stream));
+        dev = static_cast<const NPDiagOp *>(h->optab.ptr);
+    }
+    NPDev d = dev_view(h);
+    size_t run = 0, first_op = 0;
+    for (size_t i = 0; i < gates.size();) {
+        if (run < runs.size() && runs[run].first == i) {
+            const int nops = (int)(runs[run].second - runs[run].first);
+            if (h->prec == 8)
+                hipLaunchKernelGGL(k_np_diag<double>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                                   h->stream, static_cast<cplx<double> *>(h->amps), d,
+                                   dev + first_op, nops);
+            else
+                hipLaunchKernelGGL(k_np_diag<float>, dim3(h->npt->nblk), dim3(kBlock), 0,
+                                   h->stream, static_cast<cplx<float> *>(h->amps), d,
+                                   dev + first_op, nops);
+            QH_CHECK(hipGetLastError());
+            first_op += (size_t)nops;
+            i = runs[run].second;
+            run++;
+            continue;
+        }
+        np_launch_gate(h, gates[i]);
+        i++;
+    }
 }
 
 double np_prob(qslhip_state *h, unsigned t, unsigned outcome)

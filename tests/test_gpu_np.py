@@ -169,3 +169,41 @@ def test_np_n32_k16_properties():
     amp = q.getAmps(O.rank(x0), 1)[0]
     assert abs(amp - 1) < 1e-12
     assert abs(q.norm() - 1) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,k", [(10, 4), (20, 10)])
+def test_np_diagonal_runs_in_one_pass(dtype, n, k):
+    # runs of diagonal gates (phase, pauliZ, rotateZ, controlZ, controlPhase,
+    # controlRotateZ) go through one pass each, every amplitude taking the
+    # run's factors in program order: bit-identical to one kernel per gate
+    rng = np.random.default_rng(500 + n + k)
+    diag1, diag2 = [2, 4, 7], [18, 21, 22]
+    seq = []
+    for layer in range(4):
+        for q0 in range(layer % 2, n - 1, 2):  # adjacent bonds, as in config 3
+            seq.append((int(rng.choice([26, 27])), q0, q0 + 1, 0, _angle(rng, dtype)))
+        for _ in range(3 * n):
+            if rng.random() < 0.5:
+                seq.append((int(rng.choice(diag1)), int(rng.integers(n)), 0, 0, _angle(rng, dtype)))
+            else:
+                a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+                seq.append((int(rng.choice(diag2)), a, b, 0, _angle(rng, dtype)))
+    ops = np.array(seq, OP_DTYPE)
+    st0 = random_np_state(rng, n, k, dtype)
+    fused = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    fused.setState(st0)
+    fused.apply_ops(ops)
+    per_gate = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    per_gate.setState(st0)
+    per_gate.apply_ops(ops, mode=qsl_amd.MODE_PER_GATE)
+    assert _bits_equal(fused.getAmps(), per_gate.getAmps())
+    if n <= 10:
+        want = st0.copy()
+        for o in ops:
+            kind, a, b, ang = int(o["kind"]), int(o["q0"]), int(o["q1"]), float(o["angle"])
+            if kind >= 16:
+                O.np_gate2(want, k, kind, a, b, ang)
+            else:
+                O.np_gate1(want, k, kind, a, ang)
+        assert _bits_equal(fused.getState(), want)
