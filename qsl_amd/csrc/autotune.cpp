@@ -112,6 +112,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "reg_min_qubits") *value = t.reg_min_qubits;
         else if (k == "jit") *value = t.jit;
         else if (k == "jit_waves") *value = t.jit_waves;
+        else if (k == "np_bonds") *value = t.np_bonds;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
         else if (k == "run_sums_used") *value = g_run_sums_used;
@@ -158,6 +159,9 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "jit") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit must be 0, 1 or 2");
             t.jit = value;
+        } else if (k == "np_bonds") {
+            QH_REQUIRE(value >= 1 && value <= 4, QSLHIP_EINVAL, "np_bonds must be in [1, 4]");
+            t.np_bonds = value;
         } else if (k == "jit_waves") {
             QH_REQUIRE(value >= 0 && value <= 8, QSLHIP_EINVAL, "jit_waves must be in [0, 8]");
             t.jit_waves = value;

@@ -203,6 +203,8 @@ struct Tuning {
     int reg_min_qubits = 18;     // smaller states: LDS tiles (latency bound there)
     int jit = 1;                 // register-tile passes compiled at run time: 0 never,
                                  // 1 once a pass structure repeats, 2 always
+    int np_bonds = 1;            // NP: swap-like gates on disjoint pairs per pass (1: one
+                                 // each; grouping measured slower for config 3)
     int jit_waves = 0;           // compiled passes: register budget for this many
                                  // waves per SIMD (0: the compiler's choice)
 };

@@ -225,6 +225,83 @@ __global__ __launch_bounds__(kBlock) void k_np_diag(cplx<T> *__restrict__ amp, N
     }
 }
 
+// Up to kBondMax swap-like gates (npRotateX/Y, npHadamard, swap) on
+// pairwise disjoint qubit pairs, in program order.
+constexpr int kBondMax = 4;
+struct NPBonds {
+    int m;
+    int body[kBondMax];
+    uint32_t q0[kBondMax], q1[kBondMax];
+    double c[kBondMax], s[kBondMax];
+};
+
+// The gates of NPBonds in one pass.  A gate on (q0, q1) pairs each string
+// with bit q0 set and q1 clear (its owner) with the string that has the two
+// bits exchanged (k_np_gate's PC_SWAPLIKE).  With disjoint pairs the strings
+// reachable from x by exchanging the bits of any subset of its "active"
+// pairs (the two bits differ) form an orbit of 2^a strings that no other
+// gate of the run leaves.  The thread whose string owns every active pair
+// gathers the orbit, applies the gates in program order with the same
+// pair_update as one launch per gate, and scatters it back: bit-identical.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_bonds(cplx<T> *__restrict__ amp, NPDev d,
+                                                     NPBonds B)
+{
+    const Seg g = seg_of(d);
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+#pragma unroll 1
+    for (int e = 0; e < kItems; e++) {
+        uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        if (i >= g.size || i >= g.i0 + kChunk) continue;
+        const uint64_t x = hix | d.lo_list[g.cstart + i];
+        unsigned active = 0;
+        bool rep = true;
+#pragma unroll
+        for (int j = 0; j < kBondMax; j++) {
+            if (j >= B.m) break;
+            const bool a0 = (x >> B.q0[j]) & 1ull, a1 = (x >> B.q1[j]) & 1ull;
+            if (a0 != a1) {
+                active |= 1u << j;
+                if (!a0) rep = false;  // another orbit member owns this pair
+            }
+        }
+        if (!rep || !active) continue;
+        cplx<T> v[1 << kBondMax];
+        uint64_t rk[1 << kBondMax];
+        const int na = 1 << B.m;
+#pragma unroll
+        for (int sub = 0; sub < (1 << kBondMax); sub++) {
+            if (sub >= na || (sub & ~active)) continue;
+            uint64_t y = x;
+#pragma unroll
+            for (int j = 0; j < kBondMax; j++)
+                if ((sub >> j) & 1) y ^= (1ull << B.q0[j]) | (1ull << B.q1[j]);
+            rk[sub] = sub ? np_rank(d, y) : (g.off + i);
+            v[sub] = amp[rk[sub]];
+        }
+#pragma unroll
+        for (int j = 0; j < kBondMax; j++) {
+            if (j >= B.m || !((active >> j) & 1)) continue;
+            const T c = (T)B.c[j], s = (T)B.s[j];
+#pragma unroll
+            for (int sub = 0; sub < (1 << kBondMax); sub++) {
+                if (sub >= na || (sub & ~active) || ((sub >> j) & 1)) continue;
+                // v[sub] owns pair j (bits q0 = 1, q1 = 0), its partner is sub | bit j
+                const int par = sub | (1 << j);
+                cplx<T> a = v[sub], b = v[par];
+                pair_update<T>(B.body[j], a, b, c, s);
+                v[sub] = a;
+                v[par] = b;
+            }
+        }
+#pragma unroll
+        for (int sub = 0; sub < (1 << kBondMax); sub++) {
+            if (sub >= na || (sub & ~active)) continue;
+            amp[rk[sub]] = v[sub];
+        }
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -538,31 +615,59 @@ static int diag_class(int kind)
     }
 }
 
+static bool bond_kind(int kind)
+{
+    return kind == QSLHIP_SWAP || kind == QSLHIP_NP_ROT_X || kind == QSLHIP_NP_ROT_Y ||
+           kind == QSLHIP_NP_HADAMARD;
+}
+
 void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
 {
-    // runs of two or more consecutive diagonal gates become one pass each
-    // (k_np_diag); the ops of all runs are staged in one copy
-    std::vector<NPDiagOp> dops;
-    std::vector<std::pair<size_t, size_t>> runs;  // [first gate, end) of each run
+    // Segments of the list: runs of two or more diagonal gates (one
+    // k_np_diag pass each, their ops staged in one copy), groups of two to
+    // kBondMax swap-like gates on disjoint qubit pairs (one k_np_bonds pass
+    // each), and single gates.
+    enum { SINGLE, DIAG, BONDS };
+    struct Part { int type; size_t first, end; };
+    std::vector<Part> parts;
     for (size_t i = 0; i < gates.size();) {
         size_t j = i;
         while (j < gates.size() && diag_class(gates[j].kind) >= 0) j++;
         if (j - i >= 2) {
-            runs.emplace_back(i, j);
-            for (size_t k = i; k < j; k++) {
-                const GateDesc &g = gates[k];
-                NPDiagOp o;
-                o.cls = diag_class(g.kind);
-                o.body = body_of(g.kind);
-                o.q0 = g.q0;
-                o.q1 = is_gate2(g.kind) ? g.q1 : g.q0;
-                o.c = g.c;
-                o.s = g.s;
-                dops.push_back(o);
-            }
+            parts.push_back({DIAG, i, j});
             i = j;
-        } else {
-            i = j > i ? j : i + 1;
+            continue;
+        }
+        uint64_t used = 0;
+        j = i;
+        const size_t gmax = (size_t)tuning().np_bonds;
+        while (j < gates.size() && j - i < gmax && bond_kind(gates[j].kind)) {
+            const uint64_t m = (1ull << gates[j].q0) | (1ull << gates[j].q1);
+            if (used & m) break;
+            used |= m;
+            j++;
+        }
+        if (j - i >= 2) {
+            parts.push_back({BONDS, i, j});
+            i = j;
+            continue;
+        }
+        parts.push_back({SINGLE, i, i + 1});
+        i++;
+    }
+    std::vector<NPDiagOp> dops;
+    for (const Part &pt : parts) {
+        if (pt.type != DIAG) continue;
+        for (size_t k = pt.first; k < pt.end; k++) {
+            const GateDesc &g = gates[k];
+            NPDiagOp o;
+            o.cls = diag_class(g.kind);
+            o.body = body_of(g.kind);
+            o.q0 = g.q0;
+            o.q1 = is_gate2(g.kind) ? g.q1 : g.q0;
+            o.c = g.c;
+            o.s = g.s;
+            dops.push_back(o);
         }
     }
     const NPDiagOp *dev = nullptr;
@@ -578,26 +683,41 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
         dev = static_cast<const NPDiagOp *>(h->optab.ptr);
     }
     NPDev d = dev_view(h);
-    size_t run = 0, first_op = 0;
-    for (size_t i = 0; i < gates.size();) {
-        if (run < runs.size() && runs[run].first == i) {
-            const int nops = (int)(runs[run].second - runs[run].first);
-            if (h->prec == 8)
-                hipLaunchKernelGGL(k_np_diag<double>, dim3(h->npt->nblk), dim3(kBlock), 0,
-                                   h->stream, static_cast<cplx<double> *>(h->amps), d,
-                                   dev + first_op, nops);
-            else
-                hipLaunchKernelGGL(k_np_diag<float>, dim3(h->npt->nblk), dim3(kBlock), 0,
-                                   h->stream, static_cast<cplx<float> *>(h->amps), d,
-                                   dev + first_op, nops);
-            QH_CHECK(hipGetLastError());
-            first_op += (size_t)nops;
-            i = runs[run].second;
-            run++;
+    const dim3 grid(h->npt->nblk), blk(kBlock);
+    size_t first_op = 0;
+    for (const Part &pt : parts) {
+        if (pt.type == SINGLE) {
+            np_launch_gate(h, gates[pt.first]);
             continue;
         }
-        np_launch_gate(h, gates[i]);
-        i++;
+        if (pt.type == DIAG) {
+            const int nops = (int)(pt.end - pt.first);
+            if (h->prec == 8)
+                hipLaunchKernelGGL(k_np_diag<double>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<double> *>(h->amps), d, dev + first_op, nops);
+            else
+                hipLaunchKernelGGL(k_np_diag<float>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<float> *>(h->amps), d, dev + first_op, nops);
+            first_op += (size_t)nops;
+        } else {
+            NPBonds B{};
+            B.m = (int)(pt.end - pt.first);
+            for (int j = 0; j < B.m; j++) {
+                const GateDesc &g = gates[pt.first + j];
+                B.body[j] = body_of(g.kind);
+                B.q0[j] = g.q0;
+                B.q1[j] = g.q1;
+                B.c[j] = g.c;
+                B.s[j] = g.s;
+            }
+            if (h->prec == 8)
+                hipLaunchKernelGGL(k_np_bonds<double>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<double> *>(h->amps), d, B);
+            else
+                hipLaunchKernelGGL(k_np_bonds<float>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<float> *>(h->amps), d, B);
+        }
+        QH_CHECK(hipGetLastError());
     }
 }
 

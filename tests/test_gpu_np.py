@@ -207,3 +207,54 @@ def test_np_diagonal_runs_in_one_pass(dtype, n, k):
             else:
                 O.np_gate1(want, k, kind, a, ang)
         assert _bits_equal(fused.getState(), want)
+
+
+@pytest.fixture
+def bond_groups():
+    saved = qsl_amd._lib.tuning_get("np_bonds")
+    qsl_amd._lib.tuning_set("np_bonds", 4)
+    yield
+    qsl_amd._lib.tuning_set("np_bonds", saved)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,k", [(12, 5), (20, 10), (24, 12)])
+def test_np_disjoint_bond_groups_in_one_pass(bond_groups, dtype, n, k):
+    # consecutive swap-like gates (npRotateX/Y, npHadamard, swap) on disjoint
+    # qubit pairs run in groups of up to four per pass, each orbit of strings
+    # gathered once: bit-identical to one kernel per gate, for pairs inside
+    # the low part, the high part and across
+    rng = np.random.default_rng(700 + n + k)
+    seq = []
+    for _ in range(40):
+        qs = rng.permutation(n)
+        for j in range(int(rng.integers(1, 6))):  # 5 pairs: a group of 4 and a single
+            a, b = int(qs[2 * j]), int(qs[2 * j + 1])
+            seq.append((int(rng.choice([25, 26, 27, 28])), a, b, 0, _angle(rng, dtype)))
+        if rng.random() < 0.3:
+            seq.append((4, int(rng.integers(n)), 0, 0, _angle(rng, dtype)))  # breaks a group
+    ops = np.array(seq, OP_DTYPE)
+    st0 = random_np_state(rng, n, k, dtype) if n <= 14 else None
+    fused = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    per_gate = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    if st0 is not None:
+        fused.setState(st0)
+        per_gate.setState(st0)
+    else:
+        total = math.comb(n, k)
+        amp = (rng.normal(size=total) + 1j * rng.normal(size=total)).astype(dtype)
+        amp /= np.linalg.norm(amp)
+        fused.setAmps(amp)
+        per_gate.setAmps(amp)
+    fused.apply_ops(ops)
+    per_gate.apply_ops(ops, mode=qsl_amd.MODE_PER_GATE)
+    assert _bits_equal(fused.getAmps(), per_gate.getAmps())
+    if st0 is not None:
+        want = st0.copy()
+        for o in ops:
+            kind, a, b, ang = int(o["kind"]), int(o["q0"]), int(o["q1"]), float(o["angle"])
+            if kind >= 16:
+                O.np_gate2(want, k, kind, a, b, ang)
+            else:
+                O.np_gate1(want, k, kind, a, ang)
+        assert _bits_equal(fused.getState(), want)
