@@ -166,8 +166,9 @@ namespace qslhip {
 // gate depends on each earlier gate sharing a qubit, back to the last
 // non-diagonal gate on that qubit.  A pass takes, in index order, every
 // ready gate whose non-diagonal targets are already tile qubits; when none
-// is left it grows the tile by the earliest ready gate that still fits, and
-// it closes when no ready gate fits.  The result equals the program-order
+// is left it grows the tile by a ready gate that still fits (the one adding
+// the fewest qubits no earlier tile held, the earliest of those), and it
+// closes when no ready gate fits.  The result equals the program-order
 // product in exact arithmetic; in floating point the amplitudes differ by
 // rounding only.  `order` receives the gate index of every TileOp.
 void plan_tile_passes_reorder(unsigned n, const std::vector<GateDesc> &gates, int logt,
@@ -208,6 +209,11 @@ void plan_tile_passes_reorder(unsigned n, const std::vector<GateDesc> &gates, in
     size_t remaining = m;
     std::vector<std::vector<uint32_t>> takes;
     std::vector<uint64_t> masks;
+    // Qubits some earlier tile held.  A circuit applied to a basis state is
+    // zero outside the groups that agree with it on every other qubit, and
+    // its passes skip those groups: growing the tile with the qubits the
+    // earlier tiles already held keeps the later passes sparse longer.
+    uint64_t support = base;
     while (remaining) {
         uint64_t mask = base;
         std::vector<uint32_t> take;
@@ -224,19 +230,28 @@ void plan_tile_passes_reorder(unsigned n, const std::vector<GateDesc> &gates, in
                 for (uint32_t s : succ[i])
                     if (--npred[s] == 0) ready.insert(s);  // s > i: met later in this sweep
             }
-            bool grew = false;
+            // grow by the ready gate that fits and brings the fewest qubits
+            // no earlier tile held, the earliest of those
+            int best = -1, best_new = 65;
             for (uint32_t i : ready) {
-                if (__builtin_popcountll(mask | need[i]) <= K) {
-                    mask |= need[i];
-                    grew = true;
-                    break;
+                if (__builtin_popcountll(mask | need[i]) > K) continue;
+                const int nw = __builtin_popcountll(need[i] & ~mask & ~support);
+                if (nw < best_new) {
+                    best_new = nw;
+                    best = (int)i;
+                    if (nw == 0) break;
                 }
             }
-            if (!grew) break;
+            if (best < 0) break;
+            mask |= need[best];
         }
         QH_REQUIRE(!take.empty(), QSLHIP_EINVAL, "tile too narrow for a gate's targets");
+        // top up with the lowest qubits an earlier tile held, then the lowest
         uint64_t mk = mask;
+        for (unsigned q = 0; q < n && __builtin_popcountll(mk) < K; q++)
+            if ((support >> q) & 1ull) mk |= 1ull << q;
         for (unsigned q = 0; q < n && __builtin_popcountll(mk) < K; q++) mk |= 1ull << q;
+        support |= mk;
         takes.push_back(std::move(take));
         masks.push_back(mk);
     }
