@@ -29,7 +29,7 @@ def _angle(rng, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [3, 12])
+@pytest.mark.parametrize("n", [3, 12, 16])  # 16: the lanes' items 32/64 KiB apart
 def test_gate1_every_kind_every_target(dtype, n):
     rng = np.random.default_rng(100 + n)
     st0 = random_state(rng, n, dtype)
@@ -47,7 +47,7 @@ def test_gate1_every_kind_every_target(dtype, n):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [3, 11])
+@pytest.mark.parametrize("n", [3, 11, 16])
 def test_gate2_every_kind_every_pair(dtype, n):
     rng = np.random.default_rng(200 + n)
     st0 = random_state(rng, n, dtype)
@@ -56,7 +56,8 @@ def test_gate2_every_kind_every_pair(dtype, n):
     if n > 6:  # all pairs touching the edges, and a sample of the rest
         idx = rng.choice(len(pairs), 24, replace=False)
         pairs = sorted(set([pairs[i] for i in idx] + [(0, n - 1), (n - 1, 0),
-                                                       (0, 1), (1, 0), (n - 2, n - 1)]))
+                                                       (0, 1), (1, 0), (n - 2, n - 1)]
+                           + ([(11, 12), (3, 11), (12, 15), (13, 14)] if n > 15 else [])))
     for kind in GATE2:
         for a, b in pairs:
             ang = _angle(rng, dtype) if kind in HAS_ANGLE else 0.0
