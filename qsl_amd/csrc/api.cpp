@@ -298,6 +298,10 @@ int qslhip_destroy(qslhip_t h)
         h->pin.release();
         if (h->amps) (void)hipFree(h->amps);
         if (h->staged) (void)hipEventDestroy(h->staged);
+        if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
+        for (hipEvent_t e : h->xev)
+            if (e) (void)hipEventDestroy(e);
+        if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
         if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
         delete h;
     });

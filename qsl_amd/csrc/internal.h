@@ -115,6 +115,9 @@ struct qslhip_state {
     qslhip::DevBuf optab;    // tile-pass op tables
     qslhip::PinnedBuf pin;   // pinned host staging
     hipEvent_t staged = nullptr;  // last copy out of `pin` has completed
+    // sampling: host copies on a stream of their own, overlapping kernels
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t xev[5] = {};
     qslhip::NPTables *npt = nullptr;
     size_t amp_bytes() const { return prec == 8 ? 16 : 8; }
 };

@@ -610,14 +610,33 @@ void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *ou
     h->shots.ensure(nshots * 16);
     double *du = static_cast<double *>(h->shots.ptr);
     uint64_t *dout = reinterpret_cast<uint64_t *>(du + nshots);
+    if (!h->copy_stream) {
+        QH_CHECK(hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+        for (hipEvent_t &e : h->xev) QH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    // The uniforms go up on the copy stream while the sums kernels run (they
+    // are queued first: a copy from pageable memory may hold the host), and
+    // each quarter of the outcomes comes down while the next is computed.
     QH_CHECK(hipMemcpyAsync(du, u, nshots * sizeof(double), hipMemcpyHostToDevice,
-                            h->stream));
-    hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(nshots, kWaves)), dim3(kBlock), 0, h->stream,
-                       st, namps, pre, nchunks, subs, du, nshots, (int)strict, dout);
-    QH_CHECK(hipGetLastError());
-    QH_CHECK(hipMemcpyAsync(out, dout, nshots * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                            h->stream));
-    QH_CHECK(hipStreamSynchronize(h->stream));
+                            h->copy_stream));
+    QH_CHECK(hipEventRecord(h->xev[0], h->copy_stream));
+    QH_CHECK(hipStreamWaitEvent(h->stream, h->xev[0], 0));
+    const uint64_t parts = nshots >= (1u << 18) ? 4 : 1;
+    const uint64_t per = (nshots + parts - 1) / parts;
+    for (uint64_t c = 0; c < parts; c++) {
+        const uint64_t c0 = c * per, cn = c0 + per < nshots ? per : nshots - c0;
+        hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(cn, kWaves)), dim3(kBlock), 0, h->stream,
+                           st, namps, pre, nchunks, subs, du + c0, cn, (int)strict, dout + c0);
+        QH_CHECK(hipGetLastError());
+        QH_CHECK(hipEventRecord(h->xev[1 + c], h->stream));
+    }
+    for (uint64_t c = 0; c < parts; c++) {
+        const uint64_t c0 = c * per, cn = c0 + per < nshots ? per : nshots - c0;
+        QH_CHECK(hipStreamWaitEvent(h->copy_stream, h->xev[1 + c], 0));
+        QH_CHECK(hipMemcpyAsync(out + c0, dout + c0, cn * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, h->copy_stream));
+    }
+    QH_CHECK(hipStreamSynchronize(h->copy_stream));
 }
 
 }  // namespace
