@@ -330,6 +330,7 @@ struct BPass {
     uint64_t tile_mask;
     uint32_t top_first, top_count;  // into the pass-op table
     uint32_t crun, pad;
+    uint32_t seg_first, seg_count;  // into the segment table
 };
 
 struct BOp {
@@ -342,6 +343,7 @@ struct BDev {
     const BCirc *circ;
     const BPass *pass;
     const BOp *ops;
+    const uint32_t *segs;
     const qslhip_pauli *terms;
     const double *angles;
     const uint32_t *order;
@@ -392,14 +394,136 @@ __device__ __forceinline__ double block_reduce(double v, double *sh)
     return r;
 }
 
-// Gradient partial of one op over the LDS tile (psi, lam), element index j
-// with group base gbase; result reduced per wave into acc[slot][wave].
-__device__ __forceinline__ void tile_grad(const cplx<float> *psi, const cplx<float> *lam, int K,
-                                          uint64_t gbase, const BOp &op, double *acc, int slot)
+// ---- segments: runs of ops applied in registers -----------------------------
+// A pass's ops are cut into segments whose non-diagonal targets lie on at
+// most two tile positions s0 < s1.  Thread t holds the four amplitudes
+// base(t) | {0, 1<<s0, 1<<s1, both} of the tile and applies every op of the
+// segment to them with the arithmetic of tile_op; diagonal factors and
+// controls are read from each element's own index.  One barrier per segment
+// instead of one per op.  Segment word: s0 | s1 << 8 | first op << 16 |
+// op count << 24.
+
+struct Block {
+    cplx<float> v[4];
+    uint32_t j[4];
+};
+
+__device__ __forceinline__ void block_index(Block &b, uint32_t t, int s0, int s1)
+{
+    const uint32_t lo0 = t & ((1u << s0) - 1);
+    const uint32_t x = ((t ^ lo0) << 1) | lo0;
+    const uint32_t lo1 = x & ((1u << s1) - 1);
+    const uint32_t base = ((x ^ lo1) << 1) | lo1;
+    b.j[0] = base;
+    b.j[1] = base | (1u << s0);
+    b.j[2] = base | (1u << s1);
+    b.j[3] = base | (1u << s0) | (1u << s1);
+}
+
+__device__ __forceinline__ bool jbit(uint32_t j, int p) { return (j >> p) & 1u; }
+
+// Apply one op to a block (same element updates as tile_op).
+__device__ __forceinline__ void block_op(Block &b, int s0, const TileOp &op, uint64_t gbase)
+{
+    const int cls = pair_class(op.kind);
+    const int body = body_of(op.kind);
+    const float c = (float)op.c, s = (float)op.s;
+    const int pa = op.pa, pb = op.pb;
+    auto gbit = [&](uint32_t q) -> bool { return (gbase >> q) & 1ull; };
+    cplx<float> *v = b.v;
+    switch (cls) {
+    case PC_PAIR1:
+        if (pa >= 0) {
+            if (pa == s0) {
+                pair_update<float>(body, v[0], v[1], c, s);
+                pair_update<float>(body, v[2], v[3], c, s);
+            } else {
+                pair_update<float>(body, v[0], v[2], c, s);
+                pair_update<float>(body, v[1], v[3], c, s);
+            }
+        } else {
+            const bool one = gbit(op.q0);
+#pragma unroll
+            for (int e = 0; e < 4; e++) rz_half<float>(one, v[e], c, s);
+        }
+        break;
+    case PC_ONE:
+        if (pa >= 0) {
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (jbit(b.j[e], pa)) one_update<float>(body, v[e], c, s);
+        } else if (gbit(op.q0)) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) one_update<float>(body, v[e], c, s);
+        }
+        break;
+    case PC_CTRL:
+        if (pa < 0 && !gbit(op.q0)) break;
+        if (pb >= 0) {
+            // pairs along pb; the control bit is the same in both members
+            if (pb == s0) {
+                if (pa < 0 || jbit(b.j[0], pa)) pair_update<float>(body, v[0], v[1], c, s);
+                if (pa < 0 || jbit(b.j[2], pa)) pair_update<float>(body, v[2], v[3], c, s);
+            } else {
+                if (pa < 0 || jbit(b.j[0], pa)) pair_update<float>(body, v[0], v[2], c, s);
+                if (pa < 0 || jbit(b.j[1], pa)) pair_update<float>(body, v[1], v[3], c, s);
+            }
+        } else {
+            const bool one = gbit(op.q1);
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (pa < 0 || jbit(b.j[e], pa)) rz_half<float>(one, v[e], c, s);
+        }
+        break;
+    case PC_DIAG2:
+        if (pa < 0 && !gbit(op.q0)) break;
+        if (pb < 0 && !gbit(op.q1)) break;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if ((pa < 0 || jbit(b.j[e], pa)) && (pb < 0 || jbit(b.j[e], pb)))
+                one_update<float>(body, v[e], c, s);
+        break;
+    case PC_SWAPLIKE:
+    case PC_FSWAP:
+        // element with the pa bit set (pb clear) first, as tile_op's j0
+        if (pa == s0) pair_update<float>(body, v[1], v[2], c, s);
+        else          pair_update<float>(body, v[2], v[1], c, s);
+        if (cls == PC_FSWAP) one_update<float>(QSLHIP_PAULI_Z, v[3], c, s);
+        break;
+    default:
+        break;
+    }
+}
+
+// Sum of gen_term over a block's elements; the partner of element e is
+// element e ^ LM (a compile-time register, no indexed access).  virt: a
+// diagonal generator on a qubit outside the tile, evaluated at the virtual
+// index vj with target bit 1.
+template <int LM>
+__device__ __forceinline__ double block_terms(const Block &p, const Block &l, int gen, uint64_t vj,
+                                              uint64_t tb, uint64_t cb, uint64_t ctrl_tile,
+                                              bool virt)
+{
+    double v = 0;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint64_t j = p.j[e];
+        const bool on = (j & ctrl_tile) == ctrl_tile;
+        const cplx<float> partner = p.v[e ^ LM];
+        v += gen_term<float>(gen, virt ? vj : j, l.v[e], tb, cb, on,
+                             [&](uint64_t) { return partner; });
+    }
+    return v;
+}
+
+// Gradient partial of one op over a block (psi in p, lambda in l), summed
+// per wave into acc[slot][wave].  Targets of non-diagonal generators are
+// block positions; a diagonal generator on a qubit outside the tile takes
+// its bit from the group's base index.
+__device__ __forceinline__ void block_grad(const Block &p, const Block &l, int s0, int s1,
+                                           uint64_t gbase, const BOp &op, double *acc, int slot)
 {
     const int gen = gen_of(op.kind);
-    const uint32_t nt = 1u << K;
-    // tile-index masks: targets of non-diagonal generators are tile qubits
     uint64_t tb, cb = 0, ctrl_tile = 0;
     bool ctrl_global_on = true;
     int tpos = -1;
@@ -417,25 +541,89 @@ __device__ __forceinline__ void tile_grad(const cplx<float> *psi, const cplx<flo
     }
     double v = 0;
     if (ctrl_global_on) {
-        // a diagonal generator on a non-tile target: its bit is the group's
         const bool tglobal = (gen == GEN_Z || gen == GEN_P1) && tpos < 0;
         const uint64_t gq = is_gate2_dev(op.kind) ? op.q1 : op.q0;
-        const uint64_t gbit = tglobal ? ((gbase >> gq) & 1ull) : 0;
-        auto at = [&](uint64_t j) { return psi[j]; };
-        for (uint32_t j = threadIdx.x; j < nt; j += kBBlock) {
-            bool on = (j & ctrl_tile) == ctrl_tile;
-            if (tglobal) {
-                // evaluate with the target bit folded into a virtual index bit
-                uint64_t vj = gbit ? 1ull : 0ull;
-                v += gen_term<float>(gen, vj, lam[j], 1ull, 0, on,
-                                     [&](uint64_t) { return psi[j]; });
-            } else {
-                v += gen_term<float>(gen, j, lam[j], tb, cb, on, at);
-            }
-        }
+        const uint64_t vj = tglobal ? ((gbase >> gq) & 1ull) : 0;
+        // gen_term reads psi at the element's index with tb (and cb) flipped:
+        // a block element whose local index differs by the uniform mask lm
+        const uint64_t flip = (gen == GEN_X || gen == GEN_Y) ? tb
+                            : (gen == GEN_NPX || gen == GEN_NPY) ? (tb | cb) : 0;
+        const int lm = (int)((flip >> s0) & 1ull) | (int)(((flip >> s1) & 1ull) << 1);
+        if (tglobal) v = block_terms<0>(p, l, gen, vj, 1ull, 0, ctrl_tile, true);
+        else if (lm == 0) v = block_terms<0>(p, l, gen, 0, tb, cb, ctrl_tile, false);
+        else if (lm == 1) v = block_terms<1>(p, l, gen, 0, tb, cb, ctrl_tile, false);
+        else if (lm == 2) v = block_terms<2>(p, l, gen, 0, tb, cb, ctrl_tile, false);
+        else v = block_terms<3>(p, l, gen, 0, tb, cb, ctrl_tile, false);
     }
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) acc[slot * kBWaves + (threadIdx.x >> 6)] += v;
+}
+
+// Element j of a tile of nt amplitudes, zero past its end (blocks of a
+// state of one qubit).  Selects on the value, not on the address.
+__device__ __forceinline__ cplx<float> tile_at(const cplx<float> *tile, uint32_t j, uint32_t nt)
+{
+    cplx<float> x = tile[j < nt ? j : nt - 1];
+    if (j >= nt) {
+        x.re = 0.f;
+        x.im = 0.f;
+    }
+    return x;
+}
+
+// Forward segments over a tile of nt amplitudes in LDS; ends with a barrier.
+__device__ __forceinline__ void run_segments_fwd(cplx<float> *tile, uint32_t nt, uint64_t gbase,
+                                                 const uint32_t *sseg, uint32_t nseg,
+                                                 const TileOp *sop)
+{
+    for (uint32_t sg = 0; sg < nseg; sg++) {
+        const uint32_t w = sseg[sg];
+        const int s0 = (int)(w & 255u), s1 = (int)((w >> 8) & 255u);
+        const uint32_t first = (w >> 16) & 255u, count = w >> 24;
+        Block b;
+        block_index(b, threadIdx.x, s0, s1);
+#pragma unroll
+        for (int e = 0; e < 4; e++) b.v[e] = tile_at(tile, b.j[e], nt);
+        for (uint32_t k = first; k < first + count; k++) block_op(b, s0, sop[k], gbase);
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (b.j[e] < nt) tile[b.j[e]] = b.v[e];
+        __syncthreads();
+    }
+}
+
+// Backward segments (last op first) over psi and lambda tiles: gradient
+// partials, then the inverse ops (sop holds them).  Ends with a barrier.
+__device__ __forceinline__ void run_segments_bwd(cplx<float> *psi, cplx<float> *lam, uint32_t nt,
+                                                 uint64_t gbase, const uint32_t *sseg,
+                                                 uint32_t nseg, const TileOp *sop,
+                                                 const BOp *sbop, double *acc)
+{
+    for (uint32_t sg = nseg; sg-- > 0;) {
+        const uint32_t w = sseg[sg];
+        const int s0 = (int)(w & 255u), s1 = (int)((w >> 8) & 255u);
+        const uint32_t first = (w >> 16) & 255u, count = w >> 24;
+        Block p, l;
+        block_index(p, threadIdx.x, s0, s1);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            l.j[e] = p.j[e];
+            p.v[e] = tile_at(psi, p.j[e], nt);
+            l.v[e] = tile_at(lam, p.j[e], nt);
+        }
+        for (uint32_t k = first + count; k-- > first;) {
+            if (gen_of(sbop[k].kind) != GEN_NONE) block_grad(p, l, s0, s1, gbase, sbop[k], acc, (int)k);
+            block_op(p, s0, sop[k], gbase);
+            block_op(l, s0, sop[k], gbase);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (p.j[e] < nt) {
+                psi[p.j[e]] = p.v[e];
+                lam[p.j[e]] = l.v[e];
+            }
+        __syncthreads();
+    }
 }
 
 // lambda = H psi over the whole state (global slot or LDS), returns the
@@ -474,6 +662,21 @@ __device__ __forceinline__ double apply_h(const cplx<float> *psi, cplx<float> *l
     return block_reduce(acc, sh);
 }
 
+// Stage one pass in LDS: the op records, their TileOps (angles turned into
+// cos/sin once per pass, not once per group; inverse for the backward
+// sweep) and the segment words.  Ends with a barrier.
+__device__ __forceinline__ void stage_ops(const BDev &d, const BPass &P, bool inverse,
+                                          TileOp *sop, BOp *sbop, uint32_t *sseg)
+{
+    for (uint32_t k = threadIdx.x; k < P.top_count; k += kBBlock) {
+        const BOp o = d.ops[P.top_first + k];
+        sbop[k] = o;
+        sop[k] = make_tile_op(o, d.angles[o.gop], inverse);
+    }
+    for (uint32_t k = threadIdx.x; k < P.seg_count; k += kBBlock) sseg[k] = d.segs[P.seg_first + k];
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
 {
     // LDS: psi and lambda tiles of 2^12 FP32 amplitudes (2 x 32 KiB), the
@@ -483,6 +686,9 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
     __shared__ uint64_t hoff[1 << (kTileQubits - kLowRun)];
     __shared__ double acc[kMaxPassOps * kBWaves];
     __shared__ double sh[kBWaves];
+    __shared__ TileOp sop[kMaxPassOps];
+    __shared__ BOp sbop[kMaxPassOps];
+    __shared__ uint32_t sseg[kMaxPassOps];
     __shared__ uint32_t s_next;
 
     cplx<float> *gpsi = d.slots + (uint64_t)blockIdx.x * 2 * d.slot_amps;
@@ -510,29 +716,16 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
             __syncthreads();
             for (uint32_t p = 0; p < c.npass; p++) {
                 const BPass P = d.pass[c.pass_first + p];
-                for (uint32_t k = 0; k < P.top_count; k++) {
-                    const BOp o = d.ops[P.top_first + k];
-                    tile_op<float>(spsi, (int)c.n, 0, make_tile_op(o, d.angles[o.gop], false));
-                    __syncthreads();
-                }
+                stage_ops(d, P, false, sop, sbop, sseg);
+                run_segments_fwd(spsi, (uint32_t)namps, 0, sseg, P.seg_count, sop);
             }
             double e = apply_h(spsi, slam, namps, d.terms + c.term_first, (int)c.nterm, sh);
             if (threadIdx.x == 0) d.energy[d.order[qi]] = e;
             for (uint32_t p = c.npass; p-- > 0;) {
                 const BPass P = d.pass[c.pass_first + p];
                 for (uint32_t i = threadIdx.x; i < kMaxPassOps * kBWaves; i += kBBlock) acc[i] = 0;
-                __syncthreads();
-                for (uint32_t k = P.top_count; k-- > 0;) {
-                    const BOp o = d.ops[P.top_first + k];
-                    if (gen_of(o.kind) != GEN_NONE) {
-                        tile_grad(spsi, slam, (int)c.n, 0, o, acc, (int)k);
-                        __syncthreads();
-                    }
-                    TileOp inv = make_tile_op(o, d.angles[o.gop], true);
-                    tile_op<float>(spsi, (int)c.n, 0, inv);
-                    tile_op<float>(slam, (int)c.n, 0, inv);
-                    __syncthreads();
-                }
+                stage_ops(d, P, true, sop, sbop, sseg);
+                run_segments_bwd(spsi, slam, (uint32_t)namps, 0, sseg, P.seg_count, sop, sbop, acc);
                 for (uint32_t k = threadIdx.x; k < P.top_count; k += kBBlock) {
                     const BOp o = d.ops[P.top_first + k];
                     if (gen_of(o.kind) != GEN_NONE) {
@@ -556,7 +749,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
             const uint32_t lowm = (1u << P.crun) - 1;
             for (int m = threadIdx.x; m < (1 << (K - P.crun)); m += kBBlock)
                 hoff[m] = deposit((uint64_t)m, hi_mask);
-            __syncthreads();
+            stage_ops(d, P, false, sop, sbop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
                 for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
@@ -571,11 +764,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
                     }
                 }
                 __syncthreads();
-                for (uint32_t k = 0; k < P.top_count; k++) {
-                    const BOp o = d.ops[P.top_first + k];
-                    tile_op<float>(spsi, K, gbase, make_tile_op(o, d.angles[o.gop], false));
-                    __syncthreads();
-                }
+                run_segments_fwd(spsi, nt, gbase, sseg, P.seg_count, sop);
                 for (uint32_t e = threadIdx.x; e < nt; e += kBBlock)
                     gpsi[gbase | (e & lowm) | hoff[e >> P.crun]] = spsi[e];
                 __syncthreads();
@@ -591,7 +780,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
             for (int m = threadIdx.x; m < (1 << (K - P.crun)); m += kBBlock)
                 hoff[m] = deposit((uint64_t)m, hi_mask);
             for (uint32_t i = threadIdx.x; i < kMaxPassOps * kBWaves; i += kBBlock) acc[i] = 0;
-            __syncthreads();
+            stage_ops(d, P, true, sop, sbop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
                 for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
@@ -600,17 +789,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
                     slam[e] = glam[idx];
                 }
                 __syncthreads();
-                for (uint32_t k = P.top_count; k-- > 0;) {
-                    const BOp o = d.ops[P.top_first + k];
-                    if (gen_of(o.kind) != GEN_NONE) {
-                        tile_grad(spsi, slam, K, gbase, o, acc, (int)k);
-                        __syncthreads();
-                    }
-                    TileOp inv = make_tile_op(o, d.angles[o.gop], true);
-                    tile_op<float>(spsi, K, gbase, inv);
-                    tile_op<float>(slam, K, gbase, inv);
-                    __syncthreads();
-                }
+                run_segments_bwd(spsi, slam, nt, gbase, sseg, P.seg_count, sop, sbop, acc);
                 if (p > 0) {  // the last backward pass leaves nothing to keep
                     for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
                         uint64_t idx = gbase | (e & lowm) | hoff[e >> P.crun];
@@ -645,11 +824,47 @@ struct Batch {
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    DevBuf circ, pass, ops, terms, angles, order, queue, energy, grads, slots;
+    DevBuf circ, pass, ops, segs, terms, angles, order, queue, energy, grads, slots;
     PinnedBuf pin_angles, pin_out;
     unsigned grid = 0;
     BDev dev{};
 };
+
+// Cut a pass's ops into segments (see run_segments_fwd): a new segment
+// starts when the non-diagonal targets would span more than two positions;
+// a segment with fewer is padded with the lowest free positions.
+static void add_segments(std::vector<uint32_t> &segs, const BOp *ops, uint32_t count)
+{
+    auto need = [](const BOp &o) -> uint32_t {
+        switch (pair_class(o.kind)) {
+        case PC_PAIR1: return o.pa >= 0 ? 1u << o.pa : 0u;
+        case PC_CTRL: return o.pb >= 0 ? 1u << o.pb : 0u;
+        case PC_SWAPLIKE:
+        case PC_FSWAP:
+            QH_REQUIRE(o.pa >= 0 && o.pb >= 0, QSLHIP_EINVAL, "swap-like op outside its tile");
+            return (1u << o.pa) | (1u << o.pb);
+        default: return 0u;
+        }
+    };
+    uint32_t first = 0, req = 0;
+    auto flush = [&](uint32_t end) {
+        if (end == first) return;
+        uint32_t m = req;
+        for (unsigned p = 0; __builtin_popcount(m) < 2; p++) m |= 1u << p;
+        const uint32_t s0 = (uint32_t)__builtin_ctz(m), s1 = 31u - (uint32_t)__builtin_clz(m);
+        segs.push_back(s0 | (s1 << 8) | (first << 16) | ((end - first) << 24));
+        first = end;
+    };
+    for (uint32_t i = 0; i < count; i++) {
+        const uint32_t nd = need(ops[i]);
+        if (__builtin_popcount(req | nd) > 2) {
+            flush(i);
+            req = 0;
+        }
+        req |= nd;
+    }
+    flush(count);
+}
 
 static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
                        const qslhip_op *ops, const uint64_t *term_off,
@@ -658,6 +873,7 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
     std::vector<BCirc> circ(b.ncirc);
     std::vector<BPass> passes;
     std::vector<BOp> bops;
+    std::vector<uint32_t> segs;
     uint32_t nmax_tile = 0;
     for (uint32_t c = 0; c < b.ncirc; c++) {
         const unsigned n = nq[c];
@@ -705,6 +921,9 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
                 }
                 bops.push_back(o);
             }
+            P.seg_first = (uint32_t)segs.size();
+            add_segments(segs, bops.data() + P.top_first, P.top_count);
+            P.seg_count = (uint32_t)segs.size() - P.seg_first;
             passes.push_back(P);
         };
         if (C.lds) {
@@ -759,6 +978,7 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
     up(b.circ, circ.data(), circ.size() * sizeof(BCirc));
     up(b.pass, passes.data(), passes.size() * sizeof(BPass));
     up(b.ops, bops.data(), bops.size() * sizeof(BOp));
+    up(b.segs, segs.data(), segs.size() * sizeof(uint32_t));
     up(b.terms, terms, term_off[b.ncirc] * sizeof(qslhip_pauli));
     up(b.order, order.data(), order.size() * sizeof(uint32_t));
     b.angles.ensure(b.nops * sizeof(double) + 8);
@@ -773,6 +993,7 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
     d.circ = static_cast<const BCirc *>(b.circ.ptr);
     d.pass = static_cast<const BPass *>(b.pass.ptr);
     d.ops = static_cast<const BOp *>(b.ops.ptr);
+    d.segs = static_cast<const uint32_t *>(b.segs.ptr);
     d.terms = static_cast<const qslhip_pauli *>(b.terms.ptr);
     d.angles = static_cast<const double *>(b.angles.ptr);
     d.order = static_cast<const uint32_t *>(b.order.ptr);

@@ -18,7 +18,7 @@ import pytest
 
 import qsl_amd
 from qsl_amd import workloads as W
-from oracle.pyoracle import Oracle, OP_DTYPE
+from oracle.pyoracle import Oracle, OP_DTYPE, GATE1, GATE2
 
 pytestmark = pytest.mark.gpu
 O = Oracle()
@@ -145,6 +145,34 @@ def test_batch_vs_fp64_plain(sizes):
     for c, (n, ops, terms) in enumerate(circuits):
         e64, g64 = qsl_amd.Qubits(n).expvalGrad(ops, terms)
         assert abs(e[c] - e64) < 1e-5, (n, e[c], e64)
+        gc = g[off:off + len(ops)]
+        assert np.abs(gc - g64).max() < 1e-4, (n, np.abs(gc - g64).max())
+        off += len(ops)
+
+
+@pytest.mark.parametrize("sizes", [[1, 2, 3, 5], [8, 12], [13, 15]])
+def test_batch_every_gate_kind_vs_fp64_plain(sizes):
+    # the batch kernel applies each pass's ops in register blocks of two
+    # tile positions: every one- and two-qubit kind (controls, swaps, the
+    # number-preserving rotations, diagonal gates on qubits outside the
+    # tile), against the FP64 per-circuit path, and the energy against the
+    # oracle
+    rng = np.random.default_rng(sum(sizes))
+    circuits = []
+    for n in sizes:
+        if n == 1:
+            ops = np.zeros(40, OP_DTYPE)
+            ops["kind"] = rng.choice(GATE1, 40)
+            ops["angle"] = rng.uniform(-np.pi, np.pi, 40)
+        else:
+            ops = random_param_circuit(rng, n, 40 + 6 * n, GATE1 + GATE2)
+        circuits.append((n, ops, random_terms(rng, n, 4)))
+    e, g = qsl_amd.Batch(circuits).run()
+    off = 0
+    for c, (n, ops, terms) in enumerate(circuits):
+        e64, g64 = qsl_amd.Qubits(n).expvalGrad(ops, terms)
+        assert abs(e[c] - e64) < 1e-5, (n, e[c], e64)
+        assert abs(e64 - oracle_energy(n, ops, terms)) < 1e-12, n
         gc = g[off:off + len(ops)]
         assert np.abs(gc - g64).max() < 1e-4, (n, np.abs(gc - g64).max())
         off += len(ops)
