@@ -363,6 +363,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for rocprofv3 counter passes")
+    ap.add_argument("--profile-c4", action="store_true",
+                    help="short fixed run of the 4096-circuit batched adjoint for rocprofv3")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline 30-qubit workload")
     args = ap.parse_args()
@@ -376,6 +378,15 @@ def main():
     from qsl_amd import workloads as W
     from qsl_amd import _lib as L
 
+    if args.profile_c4:
+        # the batched variational workload: build once, then three runs of
+        # the captured graph (one persistent k_batch launch each)
+        b = qsl_amd.Batch(W.c4_batch(4096))
+        for rep in (1, 2, 3):
+            print(f"profile: batch run ({rep})", flush=True)
+            b.run()
+        print("profile: done", flush=True)
+        return
     mode = {"reorder": qsl_amd.MODE_REORDER, "fused": qsl_amd.MODE_FUSED,
             "per-gate": qsl_amd.MODE_PER_GATE}[args.mode]
     ops, stream = W.c2_random(N, NGATES)
