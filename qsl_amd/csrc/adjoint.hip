@@ -408,6 +408,15 @@ struct Block {
     uint32_t j[4];
 };
 
+// An op decoded once per pass (stage_ops): pair class, body and generator
+// looked up, cos / sin in FP32, so the per-block loop only branches on them.
+struct BlkOp {
+    int32_t cls, body, gen, kind;
+    int32_t pa, pb;
+    uint32_t q0, q1;
+    float c, s;
+};
+
 __device__ __forceinline__ void block_index(Block &b, uint32_t t, int s0, int s1)
 {
     const uint32_t lo0 = t & ((1u << s0) - 1);
@@ -423,11 +432,11 @@ __device__ __forceinline__ void block_index(Block &b, uint32_t t, int s0, int s1
 __device__ __forceinline__ bool jbit(uint32_t j, int p) { return (j >> p) & 1u; }
 
 // Apply one op to a block (same element updates as tile_op).
-__device__ __forceinline__ void block_op(Block &b, int s0, const TileOp &op, uint64_t gbase)
+__device__ __forceinline__ void block_op(Block &b, int s0, const BlkOp &op, uint64_t gbase)
 {
-    const int cls = pair_class(op.kind);
-    const int body = body_of(op.kind);
-    const float c = (float)op.c, s = (float)op.s;
+    const int cls = op.cls;
+    const int body = op.body;
+    const float c = op.c, s = op.s;
     const int pa = op.pa, pb = op.pb;
     auto gbit = [&](uint32_t q) -> bool { return (gbase >> q) & 1ull; };
     cplx<float> *v = b.v;
@@ -521,9 +530,9 @@ __device__ __forceinline__ double block_terms(const Block &p, const Block &l, in
 // block positions; a diagonal generator on a qubit outside the tile takes
 // its bit from the group's base index.
 __device__ __forceinline__ void block_grad(const Block &p, const Block &l, int s0, int s1,
-                                           uint64_t gbase, const BOp &op, double *acc, int slot)
+                                           uint64_t gbase, const BlkOp &op, double *acc, int slot)
 {
-    const int gen = gen_of(op.kind);
+    const int gen = op.gen;
     uint64_t tb, cb = 0, ctrl_tile = 0;
     bool ctrl_global_on = true;
     int tpos = -1;
@@ -574,7 +583,7 @@ __device__ __forceinline__ cplx<float> tile_at(const cplx<float> *tile, uint32_t
 // Forward segments over a tile of nt amplitudes in LDS; ends with a barrier.
 __device__ __forceinline__ void run_segments_fwd(cplx<float> *tile, uint32_t nt, uint64_t gbase,
                                                  const uint32_t *sseg, uint32_t nseg,
-                                                 const TileOp *sop)
+                                                 const BlkOp *sop)
 {
     for (uint32_t sg = 0; sg < nseg; sg++) {
         const uint32_t w = sseg[sg];
@@ -596,8 +605,7 @@ __device__ __forceinline__ void run_segments_fwd(cplx<float> *tile, uint32_t nt,
 // partials, then the inverse ops (sop holds them).  Ends with a barrier.
 __device__ __forceinline__ void run_segments_bwd(cplx<float> *psi, cplx<float> *lam, uint32_t nt,
                                                  uint64_t gbase, const uint32_t *sseg,
-                                                 uint32_t nseg, const TileOp *sop,
-                                                 const BOp *sbop, double *acc)
+                                                 uint32_t nseg, const BlkOp *sop, double *acc)
 {
     for (uint32_t sg = nseg; sg-- > 0;) {
         const uint32_t w = sseg[sg];
@@ -612,7 +620,7 @@ __device__ __forceinline__ void run_segments_bwd(cplx<float> *psi, cplx<float> *
             l.v[e] = tile_at(lam, p.j[e], nt);
         }
         for (uint32_t k = first + count; k-- > first;) {
-            if (gen_of(sbop[k].kind) != GEN_NONE) block_grad(p, l, s0, s1, gbase, sbop[k], acc, (int)k);
+            if (sop[k].gen != GEN_NONE) block_grad(p, l, s0, s1, gbase, sop[k], acc, (int)k);
             block_op(p, s0, sop[k], gbase);
             block_op(l, s0, sop[k], gbase);
         }
@@ -662,22 +670,38 @@ __device__ __forceinline__ double apply_h(const cplx<float> *psi, cplx<float> *l
     return block_reduce(acc, sh);
 }
 
-// Stage one pass in LDS: the op records, their TileOps (angles turned into
-// cos/sin once per pass, not once per group; inverse for the backward
-// sweep) and the segment words.  Ends with a barrier.
+// Stage one pass in LDS: its ops decoded (angles turned into cos/sin once
+// per pass, not once per group; inverse for the backward sweep) and its
+// segment words.  Ends with a barrier.
 __device__ __forceinline__ void stage_ops(const BDev &d, const BPass &P, bool inverse,
-                                          TileOp *sop, BOp *sbop, uint32_t *sseg)
+                                          BlkOp *sop, uint32_t *sseg)
 {
     for (uint32_t k = threadIdx.x; k < P.top_count; k += kBBlock) {
         const BOp o = d.ops[P.top_first + k];
-        sbop[k] = o;
-        sop[k] = make_tile_op(o, d.angles[o.gop], inverse);
+        const TileOp t = make_tile_op(o, d.angles[o.gop], inverse);
+        BlkOp b;
+        b.cls = pair_class(t.kind);
+        b.body = body_of(t.kind);
+        b.gen = gen_of(t.kind);
+        b.kind = t.kind;
+        b.pa = t.pa;
+        b.pb = t.pb;
+        b.q0 = t.q0;
+        b.q1 = t.q1;
+        b.c = (float)t.c;  // computed in FP32 by make_tile_op: exact
+        b.s = (float)t.s;
+        sop[k] = b;
     }
     for (uint32_t k = threadIdx.x; k < P.seg_count; k += kBBlock) sseg[k] = d.segs[P.seg_first + k];
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
+// At most 64 VGPRs (8 waves per SIMD): two 1024-thread workgroups per CU,
+// which the 80 KiB of LDS per workgroup allows.  The workgroups spend much
+// of their time at barriers and on dependent LDS / scalar work, and a
+// second one per CU runs 1.5x more circuits per second than the
+// 82-VGPR build, with the few values it spills in cold paths.
+__global__ __launch_bounds__(kBBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch(BDev d)
 {
     // LDS: psi and lambda tiles of 2^12 FP32 amplitudes (2 x 32 KiB), the
     // offsets of the scattered tile qubits, and the gradient accumulators.
@@ -686,8 +710,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
     __shared__ uint64_t hoff[1 << (kTileQubits - kLowRun)];
     __shared__ double acc[kMaxPassOps * kBWaves];
     __shared__ double sh[kBWaves];
-    __shared__ TileOp sop[kMaxPassOps];
-    __shared__ BOp sbop[kMaxPassOps];
+    __shared__ BlkOp sop[kMaxPassOps];
     __shared__ uint32_t sseg[kMaxPassOps];
     __shared__ uint32_t s_next;
 
@@ -716,7 +739,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
             __syncthreads();
             for (uint32_t p = 0; p < c.npass; p++) {
                 const BPass P = d.pass[c.pass_first + p];
-                stage_ops(d, P, false, sop, sbop, sseg);
+                stage_ops(d, P, false, sop, sseg);
                 run_segments_fwd(spsi, (uint32_t)namps, 0, sseg, P.seg_count, sop);
             }
             double e = apply_h(spsi, slam, namps, d.terms + c.term_first, (int)c.nterm, sh);
@@ -724,8 +747,8 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
             for (uint32_t p = c.npass; p-- > 0;) {
                 const BPass P = d.pass[c.pass_first + p];
                 for (uint32_t i = threadIdx.x; i < kMaxPassOps * kBWaves; i += kBBlock) acc[i] = 0;
-                stage_ops(d, P, true, sop, sbop, sseg);
-                run_segments_bwd(spsi, slam, (uint32_t)namps, 0, sseg, P.seg_count, sop, sbop, acc);
+                stage_ops(d, P, true, sop, sseg);
+                run_segments_bwd(spsi, slam, (uint32_t)namps, 0, sseg, P.seg_count, sop, acc);
                 for (uint32_t k = threadIdx.x; k < P.top_count; k += kBBlock) {
                     const BOp o = d.ops[P.top_first + k];
                     if (gen_of(o.kind) != GEN_NONE) {
@@ -749,7 +772,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
             const uint32_t lowm = (1u << P.crun) - 1;
             for (int m = threadIdx.x; m < (1 << (K - P.crun)); m += kBBlock)
                 hoff[m] = deposit((uint64_t)m, hi_mask);
-            stage_ops(d, P, false, sop, sbop, sseg);
+            stage_ops(d, P, false, sop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
                 for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
@@ -780,7 +803,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
             for (int m = threadIdx.x; m < (1 << (K - P.crun)); m += kBBlock)
                 hoff[m] = deposit((uint64_t)m, hi_mask);
             for (uint32_t i = threadIdx.x; i < kMaxPassOps * kBWaves; i += kBBlock) acc[i] = 0;
-            stage_ops(d, P, true, sop, sbop, sseg);
+            stage_ops(d, P, true, sop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
                 for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
@@ -789,7 +812,7 @@ __global__ __launch_bounds__(kBBlock) void k_batch(BDev d)
                     slam[e] = glam[idx];
                 }
                 __syncthreads();
-                run_segments_bwd(spsi, slam, nt, gbase, sseg, P.seg_count, sop, sbop, acc);
+                run_segments_bwd(spsi, slam, nt, gbase, sseg, P.seg_count, sop, acc);
                 if (p > 0) {  // the last backward pass leaves nothing to keep
                     for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
                         uint64_t idx = gbase | (e & lowm) | hoff[e >> P.crun];
