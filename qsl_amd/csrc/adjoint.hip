@@ -670,6 +670,105 @@ __device__ __forceinline__ double apply_h(const cplx<float> *psi, cplx<float> *l
     return block_reduce(acc, sh);
 }
 
+// One group of a tile pass over the HBM slot, forward: the first segment
+// loads its blocks straight from the slot (or writes |0...0> on the first
+// pass), the last one stores them straight back, and only the segments in
+// between go through the LDS tile.  Group element j lives at slot index
+// gbase | (j & lowm) | hoff[j >> crun].  Ends with a barrier.
+__device__ __forceinline__ void group_fwd(cplx<float> *gpsi, cplx<float> *tile,
+                                          const uint64_t *hoff, uint64_t gbase, uint32_t lowm,
+                                          uint32_t crun, bool init, const uint32_t *sseg,
+                                          uint32_t nseg, const BlkOp *sop)
+{
+    for (uint32_t sg = 0; sg < nseg; sg++) {
+        const uint32_t w = sseg[sg];
+        const int s0 = (int)(w & 255u), s1 = (int)((w >> 8) & 255u);
+        const uint32_t first = (w >> 16) & 255u, count = w >> 24;
+        Block b;
+        block_index(b, threadIdx.x, s0, s1);
+        if (sg == 0) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint64_t idx = gbase | (b.j[e] & lowm) | hoff[b.j[e] >> crun];
+                if (init) {
+                    b.v[e].re = (idx == 0) ? 1.f : 0.f;
+                    b.v[e].im = 0.f;
+                } else {
+                    b.v[e] = gpsi[idx];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) b.v[e] = tile[b.j[e]];
+        }
+        for (uint32_t k = first; k < first + count; k++) block_op(b, s0, sop[k], gbase);
+        if (sg + 1 == nseg) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) gpsi[gbase | (b.j[e] & lowm) | hoff[b.j[e] >> crun]] = b.v[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) tile[b.j[e]] = b.v[e];
+        }
+        __syncthreads();
+    }
+}
+
+// The same group, backward (last segment first): psi and lambda from the
+// slot into the first segment processed, back to the slot from the last
+// one when `store` (the final backward pass keeps nothing).
+__device__ __forceinline__ void group_bwd(cplx<float> *gpsi, cplx<float> *glam,
+                                          cplx<float> *psi, cplx<float> *lam,
+                                          const uint64_t *hoff, uint64_t gbase, uint32_t lowm,
+                                          uint32_t crun, bool store, const uint32_t *sseg,
+                                          uint32_t nseg, const BlkOp *sop, double *acc)
+{
+    for (uint32_t sg = nseg; sg-- > 0;) {
+        const uint32_t w = sseg[sg];
+        const int s0 = (int)(w & 255u), s1 = (int)((w >> 8) & 255u);
+        const uint32_t first = (w >> 16) & 255u, count = w >> 24;
+        Block p, l;
+        block_index(p, threadIdx.x, s0, s1);
+#pragma unroll
+        for (int e = 0; e < 4; e++) l.j[e] = p.j[e];
+        if (sg + 1 == nseg) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint64_t idx = gbase | (p.j[e] & lowm) | hoff[p.j[e] >> crun];
+                p.v[e] = gpsi[idx];
+                l.v[e] = glam[idx];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                p.v[e] = psi[p.j[e]];
+                l.v[e] = lam[p.j[e]];
+            }
+        }
+        for (uint32_t k = first + count; k-- > first;) {
+            if (sop[k].gen != GEN_NONE) block_grad(p, l, s0, s1, gbase, sop[k], acc, (int)k);
+            block_op(p, s0, sop[k], gbase);
+            block_op(l, s0, sop[k], gbase);
+        }
+        if (sg == 0) {
+            if (store) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const uint64_t idx = gbase | (p.j[e] & lowm) | hoff[p.j[e] >> crun];
+                    gpsi[idx] = p.v[e];
+                    glam[idx] = l.v[e];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                psi[p.j[e]] = p.v[e];
+                lam[p.j[e]] = l.v[e];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Stage one pass in LDS: its ops decoded (angles turned into cos/sin once
 // per pass, not once per group; inverse for the backward sweep) and its
 // segment words.  Ends with a barrier.
@@ -764,7 +863,7 @@ __global__ __launch_bounds__(kBBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
 
         // ---- tile passes over the workgroup's HBM slot -----------------------
         const int K = kTileQubits;
-        const uint32_t nt = 1u << K;
+        static_assert(kBBlock * 4 == 1 << kTileQubits, "one 4-amplitude block per thread");
         const uint64_t ngroups = namps >> K;
         for (uint32_t p = 0; p < c.npass; p++) {
             const BPass P = d.pass[c.pass_first + p];
@@ -775,23 +874,18 @@ __global__ __launch_bounds__(kBBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
             stage_ops(d, P, false, sop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
-                for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
-                    uint64_t idx = gbase | (e & lowm) | hoff[e >> P.crun];
-                    if (p == 0) {  // the first pass starts from |0...0>
-                        cplx<float> z;
-                        z.re = (idx == 0) ? 1.f : 0.f;
-                        z.im = 0.f;
-                        spsi[e] = z;
-                    } else {
-                        spsi[e] = gpsi[idx];
-                    }
-                }
-                __syncthreads();
-                run_segments_fwd(spsi, nt, gbase, sseg, P.seg_count, sop);
-                for (uint32_t e = threadIdx.x; e < nt; e += kBBlock)
-                    gpsi[gbase | (e & lowm) | hoff[e >> P.crun]] = spsi[e];
-                __syncthreads();
+                // the first pass starts from |0...0>
+                group_fwd(gpsi, spsi, hoff, gbase, lowm, P.crun, p == 0, sseg, P.seg_count, sop);
             }
+        }
+        if (c.npass == 0) {  // no ops: the state stays |0...0>
+            for (uint64_t i = threadIdx.x; i < namps; i += kBBlock) {
+                cplx<float> z;
+                z.re = (i == 0) ? 1.f : 0.f;
+                z.im = 0.f;
+                gpsi[i] = z;
+            }
+            __syncthreads();
         }
         double e = apply_h(gpsi, glam, namps, d.terms + c.term_first, (int)c.nterm, sh);
         if (threadIdx.x == 0) d.energy[d.order[qi]] = e;
@@ -806,21 +900,8 @@ __global__ __launch_bounds__(kBBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
             stage_ops(d, P, true, sop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
-                for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
-                    uint64_t idx = gbase | (e & lowm) | hoff[e >> P.crun];
-                    spsi[e] = gpsi[idx];
-                    slam[e] = glam[idx];
-                }
-                __syncthreads();
-                run_segments_bwd(spsi, slam, nt, gbase, sseg, P.seg_count, sop, acc);
-                if (p > 0) {  // the last backward pass leaves nothing to keep
-                    for (uint32_t e = threadIdx.x; e < nt; e += kBBlock) {
-                        uint64_t idx = gbase | (e & lowm) | hoff[e >> P.crun];
-                        gpsi[idx] = spsi[e];
-                        glam[idx] = slam[e];
-                    }
-                }
-                __syncthreads();
+                group_bwd(gpsi, glam, spsi, slam, hoff, gbase, lowm, P.crun, p > 0, sseg,
+                          P.seg_count, sop, acc);
             }
             for (uint32_t k = threadIdx.x; k < P.top_count; k += kBBlock) {
                 const BOp o = d.ops[P.top_first + k];

@@ -178,6 +178,21 @@ def test_batch_every_gate_kind_vs_fp64_plain(sizes):
         off += len(ops)
 
 
+def test_batch_circuits_without_ops():
+    # a circuit with no ops keeps |0...0>: energy <0|H|0>, no gradients --
+    # on the whole-state-in-LDS path (5 qubits) and the tile path (13),
+    # next to circuits with ops that use the same workgroup slots first
+    rng = np.random.default_rng(21)
+    circuits = []
+    for n in (13, 5, 14, 13, 6):
+        ops = np.zeros(0, OP_DTYPE) if n in (13, 5) else W.c4_ansatz(n, 2, W.Stream(n))[0]
+        circuits.append((n, ops, random_terms(rng, n, 5)))
+    e, g = qsl_amd.Batch(circuits).run()
+    for c, (n, ops, terms) in enumerate(circuits):
+        assert abs(e[c] - oracle_energy(n, ops, terms)) < 1e-5, (c, n)
+    assert len(g) == sum(len(c[1]) for c in circuits)
+
+
 def test_batch_vs_central_fd():
     n = 13  # tile-pass path (more than 12 qubits)
     ops, _ = W.c4_ansatz(n, 2, W.Stream(41))
