@@ -384,7 +384,7 @@ __device__ __forceinline__ int last_lane(bool f)
     return b ? 63 - __clzll((long long)b) : -1;
 }
 
-// One wavefront per shot.  Level 1: the first chunk c whose inclusive end
+// Search of one shot.  Level 1: the first chunk c whose inclusive end
 // pre[c + 1] reaches the shot (64-way search, three rounds for 2^18
 // chunks).  Levels 2 and 3: running sums from pre[c] over the sub-chunk
 // sums, then over the chosen sub-chunk's |a|^2; the outcome is the first
@@ -392,30 +392,20 @@ __device__ __forceinline__ int last_lane(bool f)
 // entries leave the reference's running sum unchanged, and generateDist
 // skips them), else the last entry of nonzero probability (the chunk's
 // total reached the shot, its partial sums fell short by rounding).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_sample(const cplx<T> *__restrict__ st,
-                                                   uint64_t namps,
-                                                   const double *__restrict__ pre,
-                                                   uint64_t nchunks,
-                                                   const double *__restrict__ subs,
-                                                   const double *__restrict__ u,
-                                                   uint64_t nshots, int strict,
-                                                   uint64_t *__restrict__ out)
-{
-    const int lane = threadIdx.x & 63;
-    const uint64_t shot = (uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
-    if (shot >= nshots) return;  // the whole wave
-    const double x = u[shot];
-    ShotRule hit{x, strict ? x : 0.0, strict != 0 || !(x > 0.0)};
-    // level 1: the answer is in [lo, hi]; hi is a chunk known to hit, or
-    // nchunks (none).  Lane l probes the end of the l-th of 64 equal steps.
-    uint64_t lo = 0, hi = nchunks;
-    while (lo < hi) {
+
+// One round of level 1 on [lo, hi]: hi is a chunk known to hit, or nchunks
+// (none).  Lane l probes the end of the l-th of 64 equal steps.
+struct Level1 {
+    uint64_t lo, hi, step, c;
+    __device__ void probe(int lane)
+    {
         const uint64_t span = hi - lo;
-        const uint64_t step = span <= 64 ? 1 : (span + 63) / 64;
-        uint64_t c = lo + (uint64_t)(lane + 1) * step - 1;
+        step = span <= 64 ? 1 : (span + 63) / 64;
+        c = lo + (uint64_t)(lane + 1) * step - 1;
         if (c > hi - 1) c = hi - 1;
-        const int f = first_lane(hit(pre[c + 1]));
+    }
+    __device__ void narrow(int f)
+    {
         if (f < 0) {
             lo = hi;  // the last probe is hi - 1
         } else {
@@ -424,9 +414,17 @@ __global__ __launch_bounds__(kBlock) void k_sample(const cplx<T> *__restrict__ s
             hi = cf < hi - 1 ? cf : hi - 1;
         }
     }
-    uint64_t c = lo;
-    // levels 2 and 3; a chunk whose partial sums all stay zero (its prefix
-    // moved only by rounding) hands the shot to the next chunk
+};
+
+// Levels 2 and 3 from chunk c on; a chunk whose partial sums all stay zero
+// (its prefix moved only by rounding) hands the shot to the next chunk.
+// Past the total mass: drawSample returns 0; sampleAll2 counts nothing.
+template <typename T>
+__device__ uint64_t resolve_from(const cplx<T> *__restrict__ st, uint64_t namps,
+                                 const double *__restrict__ pre, uint64_t nchunks,
+                                 const double *__restrict__ subs, const ShotRule &hit,
+                                 int strict, uint64_t c, int lane)
+{
     for (; c < nchunks; c++) {
         const double base = pre[c];
         const double sv = subs[c * kSubsPerChunk + lane];
@@ -441,11 +439,109 @@ __global__ __launch_bounds__(kBlock) void k_sample(const cplx<T> *__restrict__ s
         int e = first_lane(p > 0.0 && hit(run2));
         if (e < 0) e = last_lane(p > 0.0);
         if (e < 0) continue;
-        if (lane == 0) out[shot] = (c * kSubsPerChunk + (uint64_t)j) * kSub + (uint64_t)e;
-        return;
+        return (c * kSubsPerChunk + (uint64_t)j) * kSub + (uint64_t)e;
     }
-    // past the total mass: drawSample returns 0; sampleAll2 counts nothing
-    if (lane == 0) out[shot] = strict ? ~0ull : 0ull;
+    return strict ? ~0ull : 0ull;
+}
+
+// First chunk attempt of levels 2 and 3, split in phases so that two
+// shots' loads are in flight together: loads, sub-chunk scan, amplitude
+// load, amplitude scan.  e stays -1 when the chunk held nothing (the shot
+// then continues with resolve_from at c + 1).
+struct FirstChunk {
+    uint64_t c;
+    bool live;
+    double base, sv, run, p;
+    int j, e;
+    uint64_t i;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sample(const cplx<T> *__restrict__ st,
+                                                   uint64_t namps,
+                                                   const double *__restrict__ pre,
+                                                   uint64_t nchunks,
+                                                   const double *__restrict__ subs,
+                                                   const double *__restrict__ u,
+                                                   uint64_t nshots, int strict,
+                                                   uint64_t *__restrict__ out)
+{
+    // two shots per wavefront, searched in lockstep: each shot's arithmetic
+    // is the one-shot search above, the second shot's memory latency hides
+    // behind the first's
+    const int lane = threadIdx.x & 63;
+    const uint64_t shot = ((uint64_t)blockIdx.x * kWaves + (threadIdx.x >> 6)) * 2;
+    if (shot >= nshots) return;  // the whole wave
+    const bool two = shot + 1 < nshots;
+    ShotRule hit[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const double x = (s == 0 || two) ? u[shot + s] : 0.0;
+        hit[s] = ShotRule{x, strict ? x : 0.0, strict != 0 || !(x > 0.0)};
+    }
+    Level1 L[2];
+    L[0].lo = 0;
+    L[0].hi = nchunks;
+    L[1].lo = 0;
+    L[1].hi = two ? nchunks : 0;
+    while (L[0].lo < L[0].hi || L[1].lo < L[1].hi) {
+        double pv[2] = {0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            if (L[s].lo < L[s].hi) {
+                L[s].probe(lane);
+                pv[s] = pre[L[s].c + 1];
+            }
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            if (L[s].lo < L[s].hi) L[s].narrow(first_lane(hit[s](pv[s])));
+    }
+    FirstChunk F[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        F[s].c = L[s].lo;
+        F[s].live = (s == 0 || two) && F[s].c < nchunks;
+        F[s].j = -1;
+        F[s].e = -1;
+        F[s].base = F[s].sv = F[s].run = F[s].p = 0.0;
+        F[s].i = 0;
+        if (F[s].live) {
+            F[s].base = pre[F[s].c];
+            F[s].sv = subs[F[s].c * kSubsPerChunk + lane];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+        if (F[s].live) {
+            F[s].run = F[s].base + wave_scan_incl(F[s].sv, lane);
+            int j = first_lane(F[s].sv > 0.0 && hit[s](F[s].run));
+            if (j < 0) j = last_lane(F[s].sv > 0.0);
+            F[s].j = j;
+            if (j >= 0) {
+                F[s].base = j ? __shfl(F[s].run, j - 1, 64) : F[s].base;
+                F[s].i = (F[s].c * kSubsPerChunk + (uint64_t)j) * kSub + (uint64_t)lane;
+                F[s].p = F[s].i < namps ? sq(st[F[s].i]) : 0.0;
+            }
+        }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+        if (F[s].j >= 0) {
+            const double run2 = F[s].base + wave_scan_incl(F[s].p, lane);
+            int e = first_lane(F[s].p > 0.0 && hit[s](run2));
+            if (e < 0) e = last_lane(F[s].p > 0.0);
+            F[s].e = e;
+        }
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (s == 1 && !two) break;
+        uint64_t o;
+        if (F[s].e >= 0)
+            o = (F[s].c * kSubsPerChunk + (uint64_t)F[s].j) * kSub + (uint64_t)F[s].e;
+        else
+            o = resolve_from<T>(st, namps, pre, nchunks, subs, hit[s], strict,
+                                F[s].live ? F[s].c + 1 : nchunks, lane);
+        if (lane == 0) out[shot + s] = o;
+    }
 }
 
 template <typename T>
@@ -625,7 +721,7 @@ void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *ou
     const uint64_t per = (nshots + parts - 1) / parts;
     for (uint64_t c = 0; c < parts; c++) {
         const uint64_t c0 = c * per, cn = c0 + per < nshots ? per : nshots - c0;
-        hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(cn, kWaves)), dim3(kBlock), 0, h->stream,
+        hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(cn, 2 * kWaves)), dim3(kBlock), 0, h->stream,
                            st, namps, pre, nchunks, subs, du + c0, cn, (int)strict, dout + c0);
         QH_CHECK(hipGetLastError());
         QH_CHECK(hipEventRecord(h->xev[1 + c], h->stream));
