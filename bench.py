@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("30q FP64 gates/s + HBM GB/s; batched adjoint grads/s, "
           "FP32 vs FP64 plain")
 HBM_PEAK_GBPS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 N = 30
 NGATES = 200
 NSHOTS = 10 ** 6
@@ -303,6 +304,19 @@ def bench_c5(layers34=30, layers30=30):
     res.update({"nqubits": n, "layers": layers34, "gates": len(ops), "ms": round(ms, 1),
                 "gates_per_s": round(len(ops) / (ms * 1e-3), 1),
                 "norm_err": abs(q.norm() - 1)})
+    # matrix-core work of the plan: each dense unitary is a real GEMM with
+    # the interleaved embedding, 2^F x 2^F complex = 2^(F+1) x 2^(F+1) real,
+    # i.e. 2 * 2^(F+1) * 2 = 8 << F flops per amplitude (v_mfma_f32_32x32x2f32)
+    _, oi, _ = L.plan_dense(n, ops, F)
+    ndense = int((oi[:, 0] == 1).sum())
+    flops = ndense * (8 << F) * float(1 << n)
+    tflops = flops / (ms * 1e-3) / 1e12
+    res["mfma"] = {"dense_unitaries": ndense, "single_gate_ops": int((oi[:, 0] == 0).sum()),
+                   "flops": flops, "achieved_TFLOPs": round(tflops, 1),
+                   "peak_TFLOPs_fp32": MFMA_FP32_PEAK_TFLOPS,
+                   "utilisation": round(tflops / MFMA_FP32_PEAK_TFLOPS, 3),
+                   "note": "plan flops over the whole dense application (loads, stores and "
+                           "single-gate tile ops included in the time)"}
     # the first 6 layers on FP32 register-tile passes, reordered (every pass
     # is compiled at run time, so the comparison stays at 6 layers): the
     # first application interprets, the second compiles, the third is timed
