@@ -379,6 +379,9 @@ def main():
                     help="short fixed run for rocprofv3 counter passes")
     ap.add_argument("--profile-c4", action="store_true",
                     help="short fixed run of the 4096-circuit batched adjoint for rocprofv3")
+    ap.add_argument("--profile-c5", action="store_true",
+                    help="short fixed run of the dense MFMA path (34 qubits FP32, 3 Haar layers) "
+                         "for rocprofv3")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline 30-qubit workload")
     args = ap.parse_args()
@@ -392,6 +395,20 @@ def main():
     from qsl_amd import workloads as W
     from qsl_amd import _lib as L
 
+    if args.profile_c5:
+        # the dense fused-unitary path on matrix cores: the first 3 of the
+        # 30 Haar layers at 34 qubits, applied once from |0...0>
+        F = L.tuning_get("dense_f")
+        ops5, _ = W.c5_haar_layers(34, 3)
+        _, oi, _ = L.plan_dense(34, ops5, F)
+        nd = int((oi[:, 0] == 1).sum())
+        q = qsl_amd.Qubits(34, dtype=np.complex64)
+        print(f"profile: dense application, {len(ops5)} gates, {nd} dense unitaries, "
+              f"{nd * (8 << F) * float(1 << 34):.4g} plan flops", flush=True)
+        q.apply_ops(ops5, mode=qsl_amd.MODE_DENSE)
+        q.sync()
+        print("profile: done", flush=True)
+        return
     if args.profile_c4:
         # the batched variational workload: build once, then three runs of
         # the captured graph (one persistent k_batch launch each)
