@@ -173,7 +173,7 @@ def bench_c1():
     wall = time.perf_counter() - t0
     res = {"gates": len(ops), "gpu_gate_ms": round(gates_ms, 4),
            "gpu_wall_ms_gates_probs_shots": round(1000 * wall, 3)}
-    try:  # the reference's Default backend, serial, and parity
+    try:  # this config's CPU baseline: the reference's Default backend, serial, and parity
         from oracle import pyoracle as P
         ref = P.RefLib()
         sim = P.RefSim(ref, P.RefLib.DEFAULT, np.complex128, 16)
@@ -204,11 +204,10 @@ def bench_c3(layers=100):
     (9.62 GB): npRotateX/Y on alternating bonds + phase on every qubit."""
     import qsl_amd
     from qsl_amd import workloads as W
-    from oracle import pyoracle as P
     n, k = 32, 16
     ops, r0, _ = W.c3_np_layers(n, layers)
     q = qsl_amd.QubitsNP(n, k)
-    x0 = P.Oracle().unrank(r0, k)
+    x0 = W.colex_unrank(r0, k)
     q.setBasisState(x0)
     ev = HipEvents()
     st = stream_of(q)

@@ -101,6 +101,21 @@ def c2_random(n=30, ngates=200, stream=None):
     return np.array(ops, OP_DTYPE), s
 
 
+def colex_unrank(r, k):
+    """The weight-k basis string of rank r among weight-k strings in
+    increasing numeric order, the order of the reference's generateLookup
+    (src/utils/misc.cpp:75-90): rank(x) = sum_i C(p_i, i) over the set bits
+    p_1 < ... < p_k (the combinatorial number system)."""
+    x = 0
+    for i in range(k, 0, -1):
+        p = i - 1
+        while math.comb(p + 1, i) <= r:
+            p += 1
+        r -= math.comb(p, i)
+        x |= 1 << p
+    return x
+
+
 def c3_np_layers(n=32, layers=100, stream=None):
     """Config 3: layer l uses bonds (q, q+1) with q = l mod 2 (step 2), each
     npRotateX or npRotateY (1/2 each, angle 2*pi*u); then phase(q, 2*pi*u)
