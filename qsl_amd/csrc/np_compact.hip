@@ -195,34 +195,52 @@ __global__ __launch_bounds__(kBlock) void k_np_diag(cplx<T> *__restrict__ amp, N
 {
     const Seg g = seg_of(d);
     const uint64_t hix = (uint64_t)g.hi << d.L;
+    // all items loaded first, then the ops outer and the items inner: each
+    // op is decoded once per thread, and each amplitude still takes the
+    // run's factors in program order
+    uint64_t x[kItems];
+    cplx<T> a[kItems];
+    bool in[kItems];
 #pragma unroll
     for (int e = 0; e < kItems; e++) {
-        uint32_t i = g.i0 + e * kBlock + threadIdx.x;
-        if (i >= g.size || i >= g.i0 + kChunk) continue;
-        const uint64_t x = hix | d.lo_list[g.cstart + i];
-        const uint64_t r = g.off + i;
-        cplx<T> a = amp[r];
-        for (int j = 0; j < nops; j++) {
-            const NPDiagOp o = ops[j];
-            const bool x0 = (x >> o.q0) & 1ull, x1 = (x >> o.q1) & 1ull;
-            const T c = (T)o.c, s = (T)o.s;
-            switch (o.cls) {
-            case PC_ONE:
-                if (x0) one_update<T>(o.body, a, c, s);
-                break;
-            case PC_PAIR1:
-                rz_half<T>(x0, a, c, s);
-                break;
-            case PC_DIAG2:
-                if (x0 && x1) one_update<T>(o.body, a, c, s);
-                break;
-            default:  // PC_CTRL
-                if (x0) rz_half<T>(x1, a, c, s);
-                break;
-            }
+        const uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        in[e] = i < g.size && i < g.i0 + kChunk;
+        x[e] = 0;
+        a[e].re = a[e].im = (T)0;
+        if (in[e]) {
+            x[e] = hix | d.lo_list[g.cstart + i];
+            a[e] = amp[g.off + i];
         }
-        amp[r] = a;
     }
+    for (int j = 0; j < nops; j++) {
+        const NPDiagOp o = ops[j];
+        const T c = (T)o.c, s = (T)o.s;
+        switch (o.cls) {
+        case PC_ONE:
+#pragma unroll
+            for (int e = 0; e < kItems; e++)
+                if ((x[e] >> o.q0) & 1ull) one_update<T>(o.body, a[e], c, s);
+            break;
+        case PC_PAIR1:
+#pragma unroll
+            for (int e = 0; e < kItems; e++) rz_half<T>((x[e] >> o.q0) & 1ull, a[e], c, s);
+            break;
+        case PC_DIAG2:
+#pragma unroll
+            for (int e = 0; e < kItems; e++)
+                if (((x[e] >> o.q0) & 1ull) && ((x[e] >> o.q1) & 1ull))
+                    one_update<T>(o.body, a[e], c, s);
+            break;
+        default:  // PC_CTRL
+#pragma unroll
+            for (int e = 0; e < kItems; e++)
+                if ((x[e] >> o.q0) & 1ull) rz_half<T>((x[e] >> o.q1) & 1ull, a[e], c, s);
+            break;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < kItems; e++)
+        if (in[e]) amp[g.off + g.i0 + e * kBlock + threadIdx.x] = a[e];
 }
 
 // Up to kBondMax swap-like gates (npRotateX/Y, npHadamard, swap) on
