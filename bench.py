@@ -204,6 +204,7 @@ def bench_c3(layers=100):
     (9.62 GB): npRotateX/Y on alternating bonds + phase on every qubit."""
     import qsl_amd
     from qsl_amd import workloads as W
+    from qsl_amd import _lib as L
     n, k = 32, 16
     ops, r0, _ = W.c3_np_layers(n, layers)
     q = qsl_amd.QubitsNP(n, k)
@@ -220,13 +221,28 @@ def bench_c3(layers=100):
     ev.record(e1, st)
     ms = ev.elapsed_ms(e0, e1)
     namps = math.comb(n, k)
-    nb = sum(1 for o in ops if int(o["kind"]) in (26, 27))
+    nb = sum(1 for o in ops if int(o["kind"]) in (L.NP_ROT_X, L.NP_ROT_Y))
     nph = len(ops) - nb
+    # bytes one launch per gate would move (bonds read and write both
+    # amplitudes of C(n-2, k-1) pairs, a phase the C(n-1, k-1) strings with
+    # its bit set)
     alg = (nb * 2 * 2 * math.comb(n - 2, k - 1) + nph * 2 * math.comb(n - 1, k - 1)) * 16
+    # bytes the launched kernels move: each run of consecutive phases is one
+    # pass over the compact vector (k_np_diag), read and written once
+    runs, prev = 0, False
+    for o in ops:
+        ph = int(o["kind"]) == L.PHASE
+        runs += ph and not prev
+        prev = ph
+    kern = (nb * 2 * 2 * math.comb(n - 2, k - 1) + runs * 2 * namps) * 16
     norm = q.norm()
     return {"layers": layers, "gates": len(ops), "ms": round(ms, 2),
             "gates_per_s": round(len(ops) / (ms * 1e-3), 1),
-            "algorithmic_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "per_gate_algorithmic_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "kernel_hbm_GBps": round(kern / (ms * 1e-3) / 1e9, 1),
+            "kernel_hbm_note": "bytes of the launched kernels (phase runs fused into one pass "
+                               "each) over the time; per_gate_algorithmic counts every phase "
+                               "as its own pass, so it can exceed the HBM peak",
             "amplitudes": namps, "state_GB": round(namps * 16 / 1e9, 2),
             "norm_err": abs(norm - 1)}
 
@@ -378,6 +394,9 @@ def main():
                     help="short fixed run for rocprofv3 counter passes")
     ap.add_argument("--profile-c4", action="store_true",
                     help="short fixed run of the 4096-circuit batched adjoint for rocprofv3")
+    ap.add_argument("--profile-c3", action="store_true",
+                    help="short fixed run of the number-preserving path (32 qubits, weight 16, "
+                         "FP64, 10 layers) for rocprofv3")
     ap.add_argument("--profile-c5", action="store_true",
                     help="short fixed run of the dense MFMA path (34 qubits FP32, 3 Haar layers) "
                          "for rocprofv3")
@@ -394,6 +413,19 @@ def main():
     from qsl_amd import workloads as W
     from qsl_amd import _lib as L
 
+    if args.profile_c3:
+        # the compact number-preserving state: 10 of the 100 layers, from the
+        # same start basis string
+        n, k = 32, 16
+        ops3, r0, _ = W.c3_np_layers(n, 10)
+        q = qsl_amd.QubitsNP(n, k)
+        q.setBasisState(W.colex_unrank(r0, k))
+        q.sync()
+        print(f"profile: NP {n}q weight {k}, {len(ops3)} gates", flush=True)
+        q.apply_ops(ops3)
+        q.sync()
+        print("profile: done", flush=True)
+        return
     if args.profile_c5:
         # the dense fused-unitary path on matrix cores: the first 3 of the
         # 30 Haar layers at 34 qubits, applied once from |0...0>
