@@ -1001,8 +1001,10 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
             QH_REQUIRE(ok1 || ok2, QSLHIP_EINVAL, "bad op " + std::to_string(i));
             gates.push_back({o.kind, o.q0, is_gate2(o.kind) ? o.q1 : 0u, 0.0, 0.0});
         }
+        // ops first..first+count-1 of the applied order; ord maps that order
+        // to the circuit's (null: program order)
         auto add_pass = [&](uint64_t mask, unsigned crun, size_t first, size_t count,
-                            const std::vector<TileOp> *tops) {
+                            const std::vector<TileOp> *tops, const uint32_t *ord) {
             BPass P;
             P.tile_mask = mask;
             P.crun = crun;
@@ -1011,12 +1013,15 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
             P.top_count = (uint32_t)count;
             for (size_t i = 0; i < count; i++) {
                 BOp o;
-                const GateDesc &g = gates[first + i];
+                const size_t gi = ord ? ord[first + i] : first + i;
+                const GateDesc &g = gates[gi];
                 o.kind = g.kind;
                 o.q0 = g.q0;
                 o.q1 = g.q1;
-                o.gop = C.op_first + (uint32_t)(first + i);
+                o.gop = C.op_first + (uint32_t)gi;
                 if (tops) {
+                    QH_REQUIRE((*tops)[i].kind == g.kind, QSLHIP_EINVAL,
+                               "batch plan: tile op out of step with its gate");
                     o.pa = (*tops)[i].pa;
                     o.pb = (*tops)[i].pb;
                 } else {  // whole state in LDS: tile index == basis index
@@ -1033,13 +1038,18 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
         if (C.lds) {
             for (size_t f = 0; f < gates.size(); f += kMaxPassOps)
                 add_pass((1ull << n) - 1, n, f,
-                         std::min<size_t>(kMaxPassOps, gates.size() - f), nullptr);
+                         std::min<size_t>(kMaxPassOps, gates.size() - f), nullptr, nullptr);
         } else {
             nmax_tile = std::max(nmax_tile, n);
             std::vector<TilePass> tp;
             std::vector<TileOp> to;
-            plan_tile_passes(n, 4, gates, kTileQubits, kLowRun, tp, to);
-            size_t g0 = 0;
+            // commuting gates regrouped into fewer passes by the gate path's
+            // support-aware reordering planner (the unitary, and so every
+            // gradient, is unchanged); each op keeps its gate's gradient slot
+            std::vector<uint32_t> ord;
+            plan_tile_passes_reorder(n, gates, kTileQubits, kLowRun, tp, to, ord);
+            QH_REQUIRE(ord.size() == gates.size() && to.size() == gates.size(), QSLHIP_EINVAL,
+                       "batch plan: reordering lost ops");
             for (const TilePass &P : tp) {
                 unsigned crun = 0;
                 while (crun < 64 && ((P.tile_mask >> crun) & 1ull)) crun++;
@@ -1050,9 +1060,8 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
                     uint32_t cnt = std::min<uint32_t>(kMaxPassOps, P.op_count - f);
                     std::vector<TileOp> slice(to.begin() + P.op_first + f,
                                               to.begin() + P.op_first + f + cnt);
-                    add_pass(P.tile_mask, crun, g0 + f, cnt, &slice);
+                    add_pass(P.tile_mask, crun, P.op_first + f, cnt, &slice, ord.data());
                 }
-                g0 += P.op_count;
             }
         }
         C.npass = (uint32_t)passes.size() - C.pass_first;
