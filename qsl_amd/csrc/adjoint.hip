@@ -865,15 +865,31 @@ __global__ __launch_bounds__(kBBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
         const int K = kTileQubits;
         static_assert(kBBlock * 4 == 1 << kTileQubits, "one 4-amplitude block per thread");
         const uint64_t ngroups = namps >> K;
+        // qubits some tile so far held: from |0...0>, an amplitude whose
+        // index has a 1 on any other qubit is still zero
+        uint64_t support = 0;
         for (uint32_t p = 0; p < c.npass; p++) {
             const BPass P = d.pass[c.pass_first + p];
             const uint64_t hi_mask = P.tile_mask & ~((1ull << P.crun) - 1);
             const uint32_t lowm = (1u << P.crun) - 1;
+            support |= P.tile_mask;
             for (int m = threadIdx.x; m < (1 << (K - P.crun)); m += kBBlock)
                 hoff[m] = deposit((uint64_t)m, hi_mask);
             stage_ops(d, P, false, sop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
+                if (gbase & ~support) {  // a zero group stays zero: store it, no ops
+                    const uint32_t t = threadIdx.x;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const uint32_t j = t + (uint32_t)e * kBBlock;
+                        cplx<float> z;
+                        z.re = 0.f;
+                        z.im = 0.f;
+                        gpsi[gbase | (j & lowm) | hoff[j >> P.crun]] = z;
+                    }
+                    continue;
+                }
                 // the first pass starts from |0...0>
                 group_fwd(gpsi, spsi, hoff, gbase, lowm, P.crun, p == 0, sseg, P.seg_count, sop);
             }
