@@ -893,6 +893,10 @@ __global__ __launch_bounds__(kBBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
                 // the first pass starts from |0...0>
                 group_fwd(gpsi, spsi, hoff, gbase, lowm, P.crun, p == 0, sseg, P.seg_count, sop);
             }
+            // zero groups end without a barrier: every thread's stores (and
+            // its reads of hoff) are done before the next pass restages
+            // hoff and the ops, and before lambda = H psi reads the slot
+            __syncthreads();
         }
         if (c.npass == 0) {  // no ops: the state stays |0...0>
             for (uint64_t i = threadIdx.x; i < namps; i += kBBlock) {
