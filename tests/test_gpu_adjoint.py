@@ -193,6 +193,31 @@ def test_batch_circuits_without_ops():
     assert len(g) == sum(len(c[1]) for c in circuits)
 
 
+def test_batch_circuits_on_a_few_qubits():
+    # gates on qubits 0..9 only (plus diagonal gates on high qubits): every
+    # forward group with a high bit set stays zero and is stored without its
+    # ops, including the last groups of each pass; each pass must finish
+    # those stores before the next one restages its offsets
+    rng = np.random.default_rng(33)
+    circuits = []
+    for n in (16, 14, 17):
+        ops = random_param_circuit(rng, 10, 120, GATE1 + GATE2)
+        diag = np.zeros(6, OP_DTYPE)
+        diag["kind"] = [4, 7, 4, 7, 4, 7]  # phase / rotateZ on high qubits
+        diag["q0"] = [n - 1, n - 2, n - 1, n - 3, n - 2, n - 1]
+        diag["angle"] = rng.uniform(-np.pi, np.pi, 6)
+        ops = np.concatenate([ops[:60], diag, ops[60:]])
+        circuits.append((n, ops, random_terms(rng, n, 4)))
+    e, g = qsl_amd.Batch(circuits).run()
+    off = 0
+    for c, (n, ops, terms) in enumerate(circuits):
+        e64, g64 = qsl_amd.Qubits(n).expvalGrad(ops, terms)
+        assert abs(e[c] - e64) < 1e-5, (n, e[c], e64)
+        gc = g[off:off + len(ops)]
+        assert np.abs(gc - g64).max() < 1e-4, (n, np.abs(gc - g64).max())
+        off += len(ops)
+
+
 def test_batch_vs_central_fd():
     n = 13  # tile-pass path (more than 12 qubits)
     ops, _ = W.c4_ansatz(n, 2, W.Stream(41))
