@@ -257,7 +257,10 @@ int qslhip_expval_grad(qslhip_t h, const qslhip_op *ops, uint64_t nops,
  * term_offsets[c+1]).  FP32 states, FP64 energies and gradients, one
  * persistent kernel captured in a hipGraph at creation.
  * qslhip_batch_run replays it with new angles (one per op, NULL keeps the
- * previous ones) and writes energies[ncirc] and grads[total ops]. */
+ * previous ones) and writes energies[ncirc] and grads[total ops], grads[i]
+ * the derivative by op i's angle.  Circuits of more than 12 qubits may apply
+ * commuting gates in another order (the same unitary; results differ by
+ * FP32 rounding only). */
 typedef struct qslhip_batch *qslhip_batch_t;
 int qslhip_batch_create(int prec, uint32_t ncirc, const uint32_t *nqubits,
                         const uint64_t *op_offsets, const qslhip_op *ops,
