@@ -256,9 +256,13 @@ int qslhip_expval_grad(qslhip_t h, const qslhip_op *ops, uint64_t nops,
  * op_offsets[c+1]) and observable terms[term_offsets[c] ..
  * term_offsets[c+1]).  FP32 states, FP64 energies and gradients, one
  * persistent kernel captured in a hipGraph at creation.
- * qslhip_batch_run replays it with new angles (one per op, NULL keeps the
- * previous ones) and writes energies[ncirc] and grads[total ops], grads[i]
- * the derivative by op i's angle.  Circuits of more than 12 qubits may apply
+ * qslhip_batch_run replays it with new angles (NULL keeps the previous
+ * ones; otherwise `angles` must hold exactly op_offsets[ncirc] doubles, one
+ * per op, since the call takes no count) and writes energies[ncirc] and
+ * grads[op_offsets[ncirc]], grads[i] the derivative by op i's angle.  The
+ * persistent grid is capped by free device memory (each workgroup owns a
+ * slot of 2 x 2^max(nqubits) FP32 amplitudes); QSLHIP_ENOMEM when not even
+ * one slot fits.  Circuits of more than 12 qubits may apply
  * commuting gates in another order (the same unitary; results differ by
  * FP32 rounding only). */
 typedef struct qslhip_batch *qslhip_batch_t;

@@ -296,9 +296,9 @@ class Oracle:
 
 class RefLib:
     """The reference library itself, compiled from qsl's sources by
-    `make -C oracle ref`.  Types: 0 Default, 1 Omp, 2 NP, 3 OmpNP."""
+    `make -C oracle ref`.  Types: 0 Default, 1 Omp, 2 NP, 3 OmpNP, 4 Resize."""
 
-    DEFAULT, OMP, NP, OMPNP = 0, 1, 2, 3
+    DEFAULT, OMP, NP, OMPNP, RESIZE = 0, 1, 2, 3, 4
 
     def __init__(self, path=REF_PATH):
         if not os.path.exists(path):
@@ -325,6 +325,21 @@ class RefLib:
         L.ref_measure_all.restype = u64
         L.ref_measure_all.argtypes = [_vp]
         L.ref_sample_all.argtypes = [_vp, u64, _vp]
+        L.ref_sample.argtypes = [_vp, U, u64, _vp]
+        L.ref_sample_all2.restype = I
+        L.ref_sample_all2.argtypes = [_vp, u64, _vp]
+        L.ref_nqubits.restype = U
+        L.ref_nqubits.argtypes = [_vp]
+        L.ref_set_state_n.restype = I
+        L.ref_set_state_n.argtypes = [_vp, _vp, U]
+        L.ref_add_qubit.restype = I
+        L.ref_add_qubit.argtypes = [_vp, U]
+        L.ref_measure_out.restype = I
+        L.ref_measure_out.argtypes = [_vp, U]
+        L.ref_postselect_out.restype = C.c_double
+        L.ref_postselect_out.argtypes = [_vp, U, U]
+        L.ref_reallocate.restype = I
+        L.ref_reallocate.argtypes = [_vp]
         L.ref_choose.restype = U
         L.ref_choose.argtypes = [U, U]
         L.ref_hamming_weight.restype = U
@@ -381,9 +396,61 @@ class RefSim:
         return self.ref.lib.ref_measure_all(self.h)
 
     def sample_all(self, nshots):
-        dense = np.zeros(1 << self.n, np.uint64)
+        dense = np.zeros(1 << self.nqubits(), np.uint64)
         self.ref.lib.ref_sample_all(self.h, nshots, _ptr(dense))
         return dense
+
+    def sample(self, t, nshots):
+        counts = np.zeros(2, np.uint64)
+        self.ref.lib.ref_sample(self.h, t, nshots, _ptr(counts))
+        return counts
+
+    def sample_all2(self, nshots):
+        dense = np.zeros(1 << self.nqubits(), np.uint64)
+        if self.ref.lib.ref_sample_all2(self.h, nshots, _ptr(dense)) != 0:
+            raise ValueError("sampleAll2 not available")
+        return dense
+
+    def nqubits(self):
+        return self.ref.lib.ref_nqubits(self.h)
+
+    # ---- Resize backend (src/qubits/resize/) ----
+    def set_state_any(self, st):
+        st = np.ascontiguousarray(st, self.dtype)
+        n = int(st.size).bit_length() - 1
+        if self.ref.lib.ref_set_state_n(self.h, _ptr(st), n) != 0:
+            raise ValueError("reference setState threw or is not a Resize object")
+
+    def get_state_any(self):
+        out = np.zeros(1 << self.nqubits(), self.dtype)
+        self.ref.lib.ref_get_state(self.h, _ptr(out))
+        return out
+
+    def add_qubit(self, t):
+        if self.ref.lib.ref_add_qubit(self.h, t) != 0:
+            raise ValueError("not a Resize object")
+
+    def measure_out(self, t):
+        return self.ref.lib.ref_measure_out(self.h, t)
+
+    def postselect_out(self, t, o):
+        return self.ref.lib.ref_postselect_out(self.h, t, o)
+
+    def reallocate(self):
+        if self.ref.lib.ref_reallocate(self.h) != 0:
+            raise ValueError("not a Resize object")
+
+
+def near_cdf_boundary(cdf, u, eps):
+    """Which uniforms lie within eps of a value of the CDF `cdf` (sorted):
+    the draws whose outcome may differ between two summation orders of the
+    same probabilities.  Sampling parity is exact outside them."""
+    cdf = np.asarray(cdf, np.float64)
+    u = np.asarray(u, np.float64)
+    j = np.searchsorted(cdf, u)
+    lo = np.abs(u - cdf[np.clip(j - 1, 0, len(cdf) - 1)])
+    hi = np.abs(u - cdf[np.clip(j, 0, len(cdf) - 1)])
+    return np.minimum(lo, hi) < eps
 
 
 def random_state(rng, n, dtype=np.complex128):

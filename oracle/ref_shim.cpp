@@ -6,8 +6,8 @@
 // Tests use it to pin the C oracle bit for bit against qsl itself, and
 // bench.py may time its Omp backend as the "reference" CPU baseline.
 //
-// Handles wrap qsl::Qubits<Type, Fp> for Type in {Default, Omp, NP, OmpNP}
-// (include/qsl/qubits.hpp:53-59) and Fp in {float, double}.  Gate kinds use
+// Handles wrap qsl::Qubits<Type, Fp> for Type in {Default, Omp, NP, OmpNP,
+// Resize} (include/qsl/qubits.hpp:53-59) and Fp in {float, double}.  Gate kinds use
 // the numbering of oracle/qsl_oracle.h.
 #include <qsl/qubits.hpp>
 #include <qsl/utils/misc.hpp>
@@ -30,11 +30,30 @@ struct Base {
     virtual int measure(unsigned t) = 0;
     virtual std::uint64_t measure_all() = 0;
     virtual void sample_all(std::uint64_t n, std::uint64_t *dense) = 0;
+    virtual void sample(unsigned t, std::uint64_t n, std::uint64_t *counts) = 0;
+    virtual int sample_all2(std::uint64_t n, std::uint64_t *dense) = 0;
     virtual unsigned nqubits() const = 0;
+    // Resize backend only (return -1 elsewhere)
+    virtual int set_state_n(const void *p, unsigned n) = 0;
+    virtual int add_qubit(unsigned t) = 0;
+    virtual int measure_out(unsigned t) = 0;
+    virtual double postselect_out(unsigned t, unsigned o) = 0;
+    virtual int reallocate() = 0;
 };
 
 template <typename Sim>
 concept HasFull = qsl::HasNonNPGates<Sim>;
+
+template <typename Sim>
+concept HasResize = requires(Sim q, unsigned t) {
+    q.addQubit(t);
+    q.measureOut(t);
+    q.postselectOut(t, t);
+    q.reallocateState();
+};
+
+template <typename Sim>
+concept HasSampleAll2 = requires(Sim q, std::size_t n) { q.sampleAll2(n); };
 
 template <typename Sim, typename Fp>
 struct Wrap final : Base {
@@ -109,12 +128,57 @@ struct Wrap final : Base {
         auto m = q.sampleAll(n);
         for (auto &kv : m) dense[kv.first] += kv.second;
     }
+    void sample(unsigned t, std::uint64_t n, std::uint64_t *counts) override {
+        auto v = q.sample(t, n);
+        counts[0] = v[0];
+        counts[1] = v[1];
+    }
+    int sample_all2(std::uint64_t n, std::uint64_t *dense) override {
+        if constexpr (HasSampleAll2<Sim>) {
+            auto m = q.sampleAll2(n);
+            for (auto &kv : m) dense[kv.first] += kv.second;
+            return 0;
+        }
+        return -1;
+    }
+    int set_state_n(const void *p, unsigned n) override {
+        if constexpr (HasResize<Sim>) {
+            std::size_t dim = std::size_t(1) << n;
+            std::vector<qsl::complex<Fp>> v(dim);
+            std::memcpy(v.data(), p, dim * sizeof(qsl::complex<Fp>));
+            q.setState(v);
+            return 0;
+        }
+        return -1;
+    }
+    int add_qubit(unsigned t) override {
+        if constexpr (HasResize<Sim>) {
+            q.addQubit(t);
+            return 0;
+        }
+        return -1;
+    }
+    int measure_out(unsigned t) override {
+        if constexpr (HasResize<Sim>) return q.measureOut(t);
+        return -1;
+    }
+    double postselect_out(unsigned t, unsigned o) override {
+        if constexpr (HasResize<Sim>) return static_cast<double>(q.postselectOut(t, o));
+        return -1;
+    }
+    int reallocate() override {
+        if constexpr (HasResize<Sim>) {
+            q.reallocateState();
+            return 0;
+        }
+        return -1;
+    }
 };
 
 template <qsl::Type T>
 Base *make(int prec, unsigned n, unsigned extra)
 {
-    if constexpr (T == qsl::Type::Default) {
+    if constexpr (T == qsl::Type::Default || T == qsl::Type::Resize) {
         if (prec == 4) return new Wrap<qsl::Qubits<T, float>, float>(n);
         return new Wrap<qsl::Qubits<T, double>, double>(n);
     } else {
@@ -128,7 +192,7 @@ Base *make(int prec, unsigned n, unsigned extra)
 extern "C" {
 
 // type: 0 Default, 1 Omp (extra = nthreads), 2 NP (extra = nones),
-// 3 OmpNP (extra = nones; the reference's default 4 threads)
+// 3 OmpNP (extra = nones; the reference's default 4 threads), 4 Resize
 void *ref_create(int type, int prec, unsigned n, unsigned extra)
 {
     try {
@@ -137,6 +201,7 @@ void *ref_create(int type, int prec, unsigned n, unsigned extra)
         case 1: return make<qsl::Type::Omp>(prec, n, extra);
         case 2: return make<qsl::Type::NP>(prec, n, extra);
         case 3: return make<qsl::Type::OmpNP>(prec, n, extra);
+        case 4: return make<qsl::Type::Resize>(prec, n, extra);
         default: return nullptr;
         }
     } catch (...) {
@@ -182,6 +247,30 @@ void ref_sample_all(void *h, std::uint64_t n, std::uint64_t *dense)
 {
     static_cast<Base *>(h)->sample_all(n, dense);
 }
+void ref_sample(void *h, unsigned t, std::uint64_t n, std::uint64_t *counts)
+{
+    static_cast<Base *>(h)->sample(t, n, counts);
+}
+int ref_sample_all2(void *h, std::uint64_t n, std::uint64_t *dense)
+{
+    return static_cast<Base *>(h)->sample_all2(n, dense);
+}
+unsigned ref_nqubits(void *h) { return static_cast<Base *>(h)->nqubits(); }
+int ref_set_state_n(void *h, const void *p, unsigned n)
+{
+    try {
+        return static_cast<Base *>(h)->set_state_n(p, n);
+    } catch (...) {
+        return -2;
+    }
+}
+int ref_add_qubit(void *h, unsigned t) { return static_cast<Base *>(h)->add_qubit(t); }
+int ref_measure_out(void *h, unsigned t) { return static_cast<Base *>(h)->measure_out(t); }
+double ref_postselect_out(void *h, unsigned t, unsigned o)
+{
+    return static_cast<Base *>(h)->postselect_out(t, o);
+}
+int ref_reallocate(void *h) { return static_cast<Base *>(h)->reallocate(); }
 
 // Utilities from src/utils/misc.cpp
 unsigned ref_choose(unsigned n, unsigned k) { return qsl::choose(n, k); }

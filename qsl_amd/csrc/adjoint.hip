@@ -878,15 +878,22 @@ __global__ __launch_bounds__(kBBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
             stage_ops(d, P, false, sop, sseg);
             for (uint64_t g = 0; g < ngroups; g++) {
                 const uint64_t gbase = deposit(g, full & ~P.tile_mask);
-                if (gbase & ~support) {  // a zero group stays zero: store it, no ops
-                    const uint32_t t = threadIdx.x;
+                if (gbase & ~support) {
+                    // A zero group stays zero.  Pass 0 stores its zeros (the
+                    // slot still holds the previous circuit); a later pass
+                    // stores nothing: each of its indices has a 1 outside
+                    // every earlier pass's support too, so it lay in a zero
+                    // group of pass 0 and no pass since has written it.
+                    if (p == 0) {
+                        const uint32_t t = threadIdx.x;
 #pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        const uint32_t j = t + (uint32_t)e * kBBlock;
-                        cplx<float> z;
-                        z.re = 0.f;
-                        z.im = 0.f;
-                        gpsi[gbase | (j & lowm) | hoff[j >> P.crun]] = z;
+                        for (int e = 0; e < 4; e++) {
+                            const uint32_t j = t + (uint32_t)e * kBBlock;
+                            cplx<float> z;
+                            z.re = 0.f;
+                            z.im = 0.f;
+                            gpsi[gbase | (j & lowm) | hoff[j >> P.crun]] = z;
+                        }
                     }
                     continue;
                 }
@@ -1103,6 +1110,20 @@ static void batch_plan(Batch &b, const uint32_t *nq, const uint64_t *op_off,
     b.grid = std::min<uint32_t>(b.ncirc, (uint32_t)(prop.multiProcessorCount * per_cu));
     if (b.grid < 1) b.grid = 1;
     const uint64_t slot_amps = nmax_tile ? (1ull << nmax_tile) : 1;
+    // Each persistent workgroup owns a slot of 2 x slot_amps FP32 amplitudes
+    // (psi and lambda).  Fewer workgroups give the same results (circuits
+    // come from a queue), so the grid shrinks to what 3/4 of the free device
+    // memory holds; with one workgroup the slot itself must fit.
+    {
+        size_t free_b = 0, total_b = 0;
+        QH_CHECK(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t slot_bytes = 2 * slot_amps * sizeof(cplx<float>);
+        const uint64_t fit = (uint64_t)(free_b / 4 * 3) / slot_bytes;
+        QH_REQUIRE(fit >= 1, QSLHIP_ENOMEM,
+                   "one batch slot of " + std::to_string(slot_bytes) +
+                       " bytes does not fit in free device memory");
+        if (fit < b.grid) b.grid = (uint32_t)fit;
+    }
 
     auto up = [&](DevBuf &buf, const void *src, size_t bytes) {
         buf.ensure(bytes ? bytes : 8);
@@ -1243,8 +1264,8 @@ int qslhip_batch_destroy(qslhip_batch_t bt)
         if (b.stream) (void)hipStreamSynchronize(b.stream);
         if (b.exec) (void)hipGraphExecDestroy(b.exec);
         if (b.graph) (void)hipGraphDestroy(b.graph);
-        for (DevBuf *d : {&b.circ, &b.pass, &b.ops, &b.terms, &b.angles, &b.order, &b.queue,
-                          &b.energy, &b.grads, &b.slots})
+        for (DevBuf *d : {&b.circ, &b.pass, &b.ops, &b.segs, &b.terms, &b.angles, &b.order,
+                          &b.queue, &b.energy, &b.grads, &b.slots})
             d->release();
         b.pin_angles.release();
         b.pin_out.release();

@@ -328,13 +328,15 @@ int qslhip_assign(qslhip_t dst, qslhip_t src)
         require_handle(dst);
         require_state_ro(src);
         if (dst == src) return;
-        dst->pending_basis = qslhip_state::kNoBasis;  // overwritten
-        dst->sums_valid = false;
-        // default.cpp:94-105 / np.cpp:119-131
+        // default.cpp:94-105 / np.cpp:119-131.  Every check comes before
+        // any change to dst, so a refused assignment leaves it as it was
+        // (a recorded basis state stays recorded).
         QH_REQUIRE(dst->n == src->n, QSLHIP_ESIZE,
                    "Cannot assign Qubits object vector from different number of qubits");
         QH_REQUIRE(dst->np == src->np && dst->prec == src->prec, QSLHIP_EINVAL,
                    "handles differ in type or precision");
+        dst->pending_basis = qslhip_state::kNoBasis;  // overwritten below
+        dst->sums_valid = false;
         if (dst->np && dst->k != src->k) {
             // the NP reference copies the full vector, whatever its weight
             np_free(dst);

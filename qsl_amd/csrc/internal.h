@@ -40,18 +40,29 @@ struct Error : std::runtime_error {
 // every shift in range.
 constexpr unsigned kMaxQubits = 40;
 
-// Device scratch that grows on demand and is released with the handle.
+// Device scratch that grows on demand.  It owns its allocation: the
+// destructor releases it, so a buffer added to a handle or batch cannot be
+// leaked by a release list that misses it.  Not copyable.
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
     void ensure(size_t b);
     void release();
 };
 
-// Host pinned scratch (results of reductions, staging of gate tables).
+// Host pinned scratch (results of reductions, staging of gate tables),
+// owned the same way.
 struct PinnedBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
     void ensure(size_t b);
     void release();
 };

@@ -248,8 +248,6 @@ class QubitsNP(Qubits):
     def __init__(self, nqubits_or_state, nones=1, dtype=np.complex128, seed=0):
         self._nones_init = nones
         super().__init__(nqubits_or_state, dtype=dtype, seed=seed)
-        if not isinstance(nqubits_or_state, np.ndarray):
-            pass
         self.name = "Qub<hipnp,%s>" % ("double" if self.prec == 8 else "float")
 
     def _create(self, lib, n, h):
@@ -307,6 +305,9 @@ class Batch:
         e = np.zeros(self.ncirc, np.float64)
         g = np.zeros(len(self.ops), np.float64)
         a = None if angles is None else np.ascontiguousarray(angles, np.float64)
+        if a is not None and (a.ndim != 1 or len(a) != len(self.ops)):
+            raise ValueError("angles must be a 1-D array of %d values (one per op), got shape %s"
+                             % (len(self.ops), a.shape))
         check(L.lib().qslhip_batch_run(self._b, None if a is None else ptr(a), ptr(e),
                                        ptr(g)))
         return e, g

@@ -323,12 +323,26 @@ static void measurement_test()
     bool same = a1.size() == c.size();
     for (size_t i = 0; same && i < c.size(); i++) same = a1[i] == c[i];
     EXPECT(same);
-    // statistical check of the self-drawn streams, as the reference does:
-    // 1000 samples, |p_est - p_true| < 0.1 (measurements.cpp:39)
-    q.seed(1234);
-    auto cnt = q.sample(2, 1000);
-    EXPECT(cnt[0] + cnt[1] == 1000);
-    EXPECT(std::fabs(cnt[0] / 1000.0 - q.prob(2, 0)) < 0.1);
+    // statistical check of the self-drawn streams, as the reference does
+    // (measurements.cpp:39), on a qubit with prob(0) = cos^2(theta/2) = 0.9,
+    // far from 1/2, so that a reversed `u < prob0` rule (which gives 0.1)
+    // fails: 2000 samples, |p_est - 0.9| < 0.05 (about 7 sigma)
+    {
+        Hip<Fp> b{3};
+        b.seed(1234);
+        b.rotateY(1, static_cast<Fp>(2 * std::acos(std::sqrt(0.9))));
+        EXPECT(std::fabs(b.prob(1, 0) - 0.9) < 10 * tol);
+        auto cnt = b.sample(1, 2000);
+        EXPECT(cnt[0] + cnt[1] == 2000);
+        EXPECT(std::fabs(cnt[0] / 2000.0 - 0.9) < 0.05);
+        // measure draws its own uniform: outcome 0 about 9 times in 10
+        std::size_t zeros = 0;
+        for (int i = 0; i < 400; i++) {
+            Hip<Fp> c{b};
+            zeros += c.measure(1) == 0;
+        }
+        EXPECT(std::fabs(zeros / 400.0 - 0.9) < 0.08);
+    }
     auto all = q.sampleAll(1000);
     size_t tot = 0;
     for (auto &[k, v] : all) tot += v;

@@ -123,6 +123,34 @@ def test_readers_see_the_recorded_state(dtype):
     assert _same(b.getState(), _basis(n, 9, dtype))
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_refused_assign_leaves_recorded_state(dtype):
+    # operator= throws on a qubit-count mismatch and leaves the object as it
+    # was (default.cpp:94-105); a fresh handle only records |0...0>, so the
+    # refusal must not drop that record
+    a = qsl_amd.Qubits(10, dtype=dtype)
+    b = qsl_amd.Qubits(12, dtype=dtype)
+    with pytest.raises(qsl_amd.QslHipError):
+        a.assign(b)
+    assert _same(a.getState(), _basis(10, 0, dtype))
+    a.setBasisState(5)
+    with pytest.raises(qsl_amd.QslHipError):
+        a.assign(b)
+    assert _same(a.getState(), _basis(10, 5, dtype))
+    # precision and type mismatches are refused the same way
+    other = np.complex64 if dtype == np.complex128 else np.complex128
+    c = qsl_amd.Qubits(10, dtype=other)
+    a.setBasisState(7)
+    with pytest.raises(qsl_amd.QslHipError):
+        a.assign(c)
+    assert _same(a.getState(), _basis(10, 7, dtype))
+    d = qsl_amd.QubitsNP(10, nones=3, dtype=dtype)
+    a.reset()
+    with pytest.raises(qsl_amd.QslHipError):
+        a.assign(d)
+    assert _same(a.getState(), _basis(10, 0, dtype))
+
+
 def test_measure_all_then_gates():
     # measureAll collapses onto the outcome (measure.cpp:110-123); the
     # collapsed basis state is recorded and synthesized by the next pass

@@ -11,18 +11,14 @@ import numpy as np
 import pytest
 
 import qsl_amd
-from oracle.pyoracle import Oracle, random_state
+from oracle.pyoracle import Oracle, random_state, near_cdf_boundary
 
 pytestmark = pytest.mark.gpu
 O = Oracle()
 PTOL = {np.complex128: 1e-12, np.complex64: 1e-5}
 
 
-def _near_boundary(cdf, u, eps):
-    j = np.searchsorted(cdf, u)
-    lo = np.abs(u - cdf[np.clip(j - 1, 0, len(cdf) - 1)])
-    hi = np.abs(u - cdf[np.clip(j, 0, len(cdf) - 1)])
-    return np.minimum(lo, hi) < eps
+_near_boundary = near_cdf_boundary
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])

@@ -15,7 +15,7 @@ import pytest
 
 import qsl_amd
 from oracle.pyoracle import (Oracle, OP_DTYPE, NP_GATE1, NP_GATE2, HAS_ANGLE,
-                             random_np_state)
+                             near_cdf_boundary, random_np_state)
 
 pytestmark = pytest.mark.gpu
 O = Oracle()
@@ -101,8 +101,13 @@ def test_np_measurement(dtype):
             assert abs(q.prob(t, o) - O.np_prob(st, k, t, o)) <= tol
     u = qsl_amd.uniforms(3, 5000)
     got = q.sampleAllOutcomes(u)
-    want = O.sample_all(st, u.astype(np.float32) if dtype == np.complex64 else u)
-    assert np.mean(got == want) > 0.999
+    uf = u.astype(np.float32) if dtype == np.complex64 else u
+    want = O.sample_all(st, uf)
+    # exact outside draws within eps of a CDF value (FP64 1e-12, FP32 1e-5)
+    _, cdf = O.gen_dist(st)
+    eps = 1e-12 if dtype == np.complex128 else 1e-5
+    ok = (got == want) | near_cdf_boundary(cdf, np.asarray(uf, np.float64), eps)
+    assert ok.all(), np.nonzero(~ok)[0][:10]
     assert all(bin(int(x)).count("1") == k for x in got)
     # measure / postselect against the full-vector oracle
     q2 = qsl_amd.QubitsNP(st)

@@ -13,7 +13,7 @@ import pytest
 
 import qsl_amd
 from qsl_amd import _lib as L
-from oracle.pyoracle import Oracle, OP_DTYPE, GATE1, GATE2
+from oracle.pyoracle import Oracle, OP_DTYPE, GATE1, GATE2, near_cdf_boundary
 
 pytestmark = pytest.mark.gpu
 O = Oracle()
@@ -32,11 +32,7 @@ def _ops(rng, n, count, dtype):
     return ops
 
 
-def _near_boundary(cdf, u, eps):
-    j = np.searchsorted(cdf, u)
-    lo = np.abs(u - cdf[np.clip(j - 1, 0, len(cdf) - 1)])
-    hi = np.abs(u - cdf[np.clip(j, 0, len(cdf) - 1)])
-    return np.minimum(lo, hi) < eps
+_near_boundary = near_cdf_boundary
 
 
 def _check_oracle(st, u, got, dtype):
@@ -130,6 +126,14 @@ def test_outcomes_equal_with_and_without_pass_sums(jit_mode, jit, mode, dtype):
     ca = _counts2(q, u)
     _drop_sums(q)
     assert np.array_equal(ca, _counts2(q, u))
+    if dtype == np.complex128:
+        # against the oracle's merge (an FP64 running sum there too): a
+        # uniform may change outcome only within 1e-12 of a running sum
+        st = q.getState()
+        want2 = O.sample_all2(st, u)
+        _, cdf = O.gen_dist(st)
+        near = int(near_cdf_boundary(cdf, u, 1e-12).sum())
+        assert np.abs(ca.astype(np.int64) - want2.astype(np.int64)).sum() <= 2 * near
     _check_oracle(q.getState(), u, q.sampleAllOutcomes(u), dtype)
     q.apply_ops(ops[:0], mode=mode)
     _check_oracle(q.getState(), u, q.sampleAllOutcomes(u), dtype)

@@ -21,7 +21,21 @@ if not os.path.exists(P.REF_PATH):
     pytest.skip("oracle/_ref not built (needs /root/reference)", allow_module_level=True)
 
 O = P.Oracle()
-REF = P.RefLib()
+
+
+class _LazyRef:
+    """The reference library, loaded by the first test that uses it (so a
+    run that deselects these tests never maps it)."""
+
+    _lib = None
+
+    def __getattr__(self, name):
+        if _LazyRef._lib is None:
+            _LazyRef._lib = P.RefLib()
+        return getattr(_LazyRef._lib, name)
+
+
+REF = _LazyRef()
 
 
 def _bits_equal(a, b):
