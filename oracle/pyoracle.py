@@ -95,6 +95,8 @@ class Oracle:
         L.orc_hamming_weight.argtypes = [u64]
         L.orc_choose.restype = u64
         L.orc_choose.argtypes = [U, U]
+        L.orc_choose_table.restype = u64
+        L.orc_choose_table.argtypes = [U, U]
         L.orc_generate_lookup.restype = S
         L.orc_generate_lookup.argtypes = [U, U, _vp]
         L.orc_np_rank.restype = u64

@@ -62,6 +62,22 @@ size_t orc_generate_lookup(unsigned len, unsigned ones, uint64_t *out)
     return m;
 }
 
+/* orc_choose(n, k) for n, k < 64, filled once when the library loads (the
+ * same values; tests compare the table with orc_choose), so that rank and
+ * unrank cost table reads rather than a loop of divisions per binomial. */
+static uint64_t g_choose[64][65];
+
+__attribute__((constructor)) static void choose_table_init(void)
+{
+    for (unsigned n = 0; n < 64; n++)
+        for (unsigned k = 0; k <= 64; k++) g_choose[n][k] = orc_choose(n, k);
+}
+
+uint64_t orc_choose_table(unsigned n, unsigned k)
+{
+    return (n < 64 && k <= 64) ? g_choose[n][k] : orc_choose(n, k);
+}
+
 /* Colex rank: sum over the set bits p_1 < ... < p_k of C(p_i, i).  Because
  * generateLookup enumerates in increasing numeric order, which is colex
  * order, lookup(n,k)[rank(x)] == x. */
@@ -72,7 +88,7 @@ uint64_t orc_np_rank(uint64_t x)
     while (x) {
         unsigned p = (unsigned)__builtin_ctzll(x);
         i++;
-        r += orc_choose(p, i);
+        r += g_choose[p][i];
         x &= x - 1;
     }
     return r;
@@ -83,9 +99,9 @@ uint64_t orc_np_unrank(uint64_t r, unsigned k)
     uint64_t x = 0;
     for (unsigned i = k; i >= 1; i--) {
         unsigned p = i - 1; /* C(i-1, i) = 0 <= r always */
-        while (orc_choose(p + 1, i) <= r) p++;
+        while (p + 1 < 64 && g_choose[p + 1][i] <= r) p++;
         x |= 1ull << p;
-        r -= orc_choose(p, i);
+        r -= g_choose[p][i];
     }
     return x;
 }

@@ -119,6 +119,7 @@ ORC_DECL(float, f)
 void orc_next(uint64_t *x);                      /* Gosper: next same popcount */
 unsigned orc_hamming_weight(uint64_t x);
 uint64_t orc_choose(unsigned n, unsigned k);     /* u64 (reference: 32-bit)   */
+uint64_t orc_choose_table(unsigned n, unsigned k); /* the same, from a table  */
 size_t orc_generate_lookup(unsigned len, unsigned ones, uint64_t *out);
 uint64_t orc_np_rank(uint64_t x);                /* colex rank of x           */
 uint64_t orc_np_unrank(uint64_t r, unsigned k);  /* inverse of orc_np_rank    */

@@ -49,6 +49,11 @@ def test_choose_kats_and_u64():
     for n in range(0, 63):
         for k in range(0, n + 1):
             assert O.choose(n, k) == math.comb(n, k)
+    # the table rank/unrank read holds the same values (k > n gives 0;
+    # n <= 62, the largest NP register, where the u64 product is exact)
+    for n in range(0, 63):
+        for k in range(0, 65):
+            assert O.lib.orc_choose_table(n, k) == math.comb(n, k)
 
 
 @pytest.mark.parametrize("n", range(1, 13))
