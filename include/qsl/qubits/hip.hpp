@@ -3,7 +3,9 @@
 // default.hpp:65-713), with the state in MI355X HBM and every gate,
 // reduction and sample run by HIP kernels of libqslhip.so.
 //
-// Member signatures are the reference's; additions are marked.  Gate
+// Member signatures are the reference's; additions are marked (the
+// gradient call expvalGrad and the batch class qsl::HipBatch are in
+// qsl/qubits/hip_grad.hpp's types).  Gate
 // arithmetic is the reference's, operation for operation, so FP64 results
 // agree with the Default backend bit for bit (and within 1e-12 by
 // contract); prob is accumulated in FP64.
@@ -11,6 +13,7 @@
 #define QSL_QUBITS_HIP_HPP
 
 #include "qsl/qubits/hip_common.hpp"
+#include "qsl/qubits/hip_grad.hpp"
 
 namespace qsl {
 
@@ -97,6 +100,30 @@ public:
         this->g2(QSLHIP_NP_ROT_Y, q1, q2, angle);
     }
     void npHadamard(unsigned q1, unsigned q2) { this->g2(QSLHIP_NP_HADAMARD, q1, q2); }
+
+    // ---- observables and adjoint gradients (no reference counterpart) --------
+    // <psi|H|psi> of the current state, H = sum of weighted Pauli strings
+    // (qsl::hip::pauli), accumulated in FP64.
+    double expval(const std::vector<hip::Pauli> &terms) const
+    {
+        double v = 0;
+        hipdetail::check(qslhip_expval(this->h_, terms.data(), terms.size(), &v));
+        return v;
+    }
+    // Applies `ops` to the current state psi0 (the object keeps U psi0) and
+    // returns E = <psi|H|psi> and dE/d(angle) of every op (0 for gates
+    // without an angle), by the adjoint reverse sweep in Fp with FP64
+    // accumulation.  Differentiable kinds: rotateX/Y/Z, phase,
+    // controlRotateX/Y/Z, controlPhase, npRotateX/Y.
+    std::pair<double, std::vector<double>> expvalGrad(const std::vector<hip::Op> &ops,
+                                                      const std::vector<hip::Pauli> &terms)
+    {
+        double e = 0;
+        std::vector<double> g(ops.size());
+        hipdetail::check(qslhip_expval_grad(this->h_, ops.data(), ops.size(), terms.data(),
+                                            terms.size(), &e, g.data()));
+        return {e, std::move(g)};
+    }
 
     // ---- sampleAll2 (measure.cpp:237-267): every index, sorted-merge rule ----
     std::map<std::size_t, std::size_t> sampleAll2(std::size_t nsamples)

@@ -180,6 +180,12 @@ public:
         return m;
     }
 
+    // Waits until every queued operation on this object has finished on the
+    // device (no reference counterpart: the reference's members return when
+    // done, these return once queued).  Timing harnesses call it around
+    // their clocks.
+    void sync() const { check(qslhip_sync(h_)); }
+
     // The opaque handle, for callers mixing the C ABI in.
     qslhip_t handle() const { return h_; }
 };

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -399,6 +400,171 @@ static void np_gate_test()
 }
 TEST(NPGatesDouble) { np_gate_test<double>(); }
 TEST(NPGatesFloat) { np_gate_test<float>(); }
+
+// ---- expectation values, adjoint gradients, batches (no reference
+// counterpart; pinned by the mathematics: parameter shift is exact for
+// rotations, central differences for the rest, all on the FP64 oracle) ----
+static double energy_of(const std::vector<double> &st, unsigned n,
+                        const std::vector<qsl::hip::Pauli> &terms)
+{
+    // P|j> = i^nY (-1)^popcount(j & z) |j ^ x>; E = sum coef Re <psi|P|psi>
+    static const double ir[4] = {1, 0, -1, 0}, ii[4] = {0, 1, 0, -1};
+    double e = 0;
+    for (const auto &t : terms) {
+        const int ny = __builtin_popcountll(t.xmask & t.zmask) & 3;
+        double acc = 0;
+        for (std::uint64_t j = 0; j < (std::uint64_t(1) << n); j++) {
+            const double sg = (__builtin_popcountll(j & t.zmask) & 1) ? -1.0 : 1.0;
+            const double ar = st[2 * j], ai = st[2 * j + 1];
+            const double cr = sg * (ir[ny] * ar - ii[ny] * ai);
+            const double ci = sg * (ir[ny] * ai + ii[ny] * ar);
+            const std::uint64_t k = j ^ t.xmask;
+            acc += st[2 * k] * cr + st[2 * k + 1] * ci;
+        }
+        e += t.coef * acc;
+    }
+    return e;
+}
+
+static double oracle_energy(unsigned n, const std::vector<qsl::hip::Op> &ops,
+                            const std::vector<qsl::hip::Pauli> &terms)
+{
+    std::vector<double> st(std::size_t(2) << n, 0.0);
+    st[0] = 1;
+    static_assert(sizeof(orc_op) == sizeof(qsl::hip::Op));
+    orc_apply_ops_d(st.data(), n, reinterpret_cast<const orc_op *>(ops.data()), ops.size());
+    return energy_of(st, n, terms);
+}
+
+// the config-4 ansatz: per layer RY, RZ on every qubit, then a CNOT ladder
+static std::vector<qsl::hip::Op> ansatz(unsigned n, unsigned layers, std::mt19937_64 &g)
+{
+    using qsl::hip::gate;
+    auto ang = [&] { return M_PI * (2 * ((g() >> 11) * 0x1.0p-53) - 1); };
+    std::vector<qsl::hip::Op> ops;
+    for (unsigned l = 0; l < layers; l++) {
+        for (unsigned q = 0; q < n; q++) ops.push_back(gate(QSLHIP_ROT_Y, q, 0, ang()));
+        for (unsigned q = 0; q < n; q++) ops.push_back(gate(QSLHIP_ROT_Z, q, 0, ang()));
+        for (unsigned q = 0; q + 1 < n; q++) ops.push_back(gate(QSLHIP_CNOT, q, q + 1));
+    }
+    return ops;
+}
+
+static std::vector<qsl::hip::Pauli> c4_terms(unsigned n)
+{
+    std::vector<qsl::hip::Pauli> t{qsl::hip::pauli(1.0, {{0, 'Z'}, {1, 'Z'}})};
+    for (unsigned q = 0; q < n; q++) t.push_back(qsl::hip::pauli(0.5, {{q, 'X'}}));
+    return t;
+}
+
+TEST(ExpvalGradDouble)
+{
+    using qsl::hip::gate;
+    const unsigned n = 6;
+    std::mt19937_64 g(41);
+    auto ops = ansatz(n, 3, g);
+    // every differentiable kind, fixed gates in between
+    ops.push_back(gate(QSLHIP_CROT_X, 2, 4, 0.7));
+    ops.push_back(gate(QSLHIP_HADAMARD, 1));
+    ops.push_back(gate(QSLHIP_CROT_Y, 5, 0, -1.1));
+    ops.push_back(gate(QSLHIP_CROT_Z, 0, 3, 2.3));
+    ops.push_back(gate(QSLHIP_PHASE, 2, 0, 0.4));
+    ops.push_back(gate(QSLHIP_CPHASE, 1, 5, -0.9));
+    ops.push_back(gate(QSLHIP_ROT_X, 4, 0, 1.9));
+    ops.push_back(gate(QSLHIP_NP_ROT_X, 1, 2, 0.3));
+    ops.push_back(gate(QSLHIP_NP_ROT_Y, 3, 4, -0.6));
+    auto terms = c4_terms(n);
+    terms.push_back(qsl::hip::pauli(-0.3, {{2, 'Y'}, {4, 'X'}}));
+    terms.push_back(qsl::hip::pauli(0.8, {{1, 'Y'}, {3, 'Z'}, {5, 'Y'}}));
+
+    Hip<double> q{n};
+    auto [e, grad] = q.expvalGrad(ops, terms);
+    EXPECT(grad.size() == ops.size());
+    EXPECT(std::fabs(e - oracle_energy(n, ops, terms)) < 1e-12);
+    // the object keeps U|0>: expval of the same observable is E again
+    EXPECT(std::fabs(q.expval(terms) - e) < 1e-12);
+    for (std::size_t i = 0; i < ops.size(); i++) {
+        const int k = ops[i].kind;
+        const bool rot = k == QSLHIP_ROT_X || k == QSLHIP_ROT_Y || k == QSLHIP_ROT_Z;
+        const bool param = rot || k == QSLHIP_PHASE || k == QSLHIP_CROT_X ||
+                           k == QSLHIP_CROT_Y || k == QSLHIP_CROT_Z || k == QSLHIP_CPHASE ||
+                           k == QSLHIP_NP_ROT_X || k == QSLHIP_NP_ROT_Y;
+        if (!param) {
+            EXPECT(grad[i] == 0.0);
+            continue;
+        }
+        auto shifted = [&](double d) {
+            auto o = ops;
+            o[i].angle += d;
+            return oracle_energy(n, o, terms);
+        };
+        if (rot) {
+            // parameter shift: exact for exp(-i t P / 2)
+            const double ps = (shifted(M_PI / 2) - shifted(-M_PI / 2)) / 2;
+            EXPECT(std::fabs(grad[i] - ps) < 1e-10);
+        } else {
+            const double h = 1e-3;
+            const double fd = (shifted(h) - shifted(-h)) / (2 * h);
+            EXPECT(std::fabs(grad[i] - fd) < 1e-6);
+        }
+    }
+    // a refused observable (qubit outside the register) throws
+    std::vector<qsl::hip::Pauli> bad{qsl::hip::pauli(1.0, {{n, 'Z'}})};
+    EXPECT_THROW(q.expval(bad), std::out_of_range);
+}
+
+TEST(BatchAgainstPlainFP64)
+{
+    std::mt19937_64 g(7);
+    std::vector<qsl::HipBatch::Circuit> cs;
+    for (unsigned n : {14u, 6u, 13u, 9u, 12u, 10u}) cs.push_back({n, ansatz(n, 4, g), c4_terms(n)});
+    qsl::HipBatch b{cs};
+    EXPECT(b.numCircuits() == cs.size());
+    std::size_t total = 0;
+    for (auto &c : cs) total += c.ops.size();
+    EXPECT(b.numOps() == total);
+    auto check = [&](const qsl::HipBatch::Result &r) {
+        for (std::size_t c = 0; c < cs.size(); c++) {
+            Hip<double> q{cs[c].nqubits};
+            auto [e64, g64] = q.expvalGrad(cs[c].ops, cs[c].terms);
+            EXPECT(std::fabs(r.energies[c] - e64) < 1e-5);
+            auto gc = r.gradient(c);
+            EXPECT(gc.size() == g64.size());
+            double m = 0;
+            for (std::size_t i = 0; i < gc.size(); i++) m = std::fmax(m, std::fabs(gc[i] - g64[i]));
+            EXPECT(m <= 1e-4);
+        }
+    };
+    check(b.run());
+    // replays with new angles (the hipGraph of the batch), each checked
+    for (int rep = 0; rep < 2; rep++) {
+        std::vector<double> angles;
+        for (auto &c : cs)
+            for (auto &o : c.ops) {
+                if (o.kind != QSLHIP_CNOT) o.angle = M_PI * (2 * ((g() >> 11) * 0x1.0p-53) - 1);
+                angles.push_back(o.angle);
+            }
+        check(b.run(angles));
+    }
+    // one circuit against the oracle's parameter shift
+    {
+        auto r = b.run();
+        const auto &c = cs[1];
+        auto gc = r.gradient(1);
+        for (std::size_t i = 0; i < c.ops.size(); i += 5) {
+            if (c.ops[i].kind == QSLHIP_CNOT) continue;
+            auto o = c.ops;
+            o[i].angle += M_PI / 2;
+            double ep = oracle_energy(c.nqubits, o, c.terms);
+            o[i].angle -= M_PI;
+            double em = oracle_energy(c.nqubits, o, c.terms);
+            EXPECT(std::fabs(gc[i] - (ep - em) / 2) <= 1e-4);
+        }
+    }
+    EXPECT_THROW(b.run(std::vector<double>(3)), std::invalid_argument);
+    qsl::HipBatch moved{std::move(b)};
+    EXPECT(moved.numCircuits() == cs.size());
+}
 
 int main(int argc, char **argv)
 {
