@@ -336,10 +336,12 @@ static void measurement_test()
         auto cnt = b.sample(1, 2000);
         EXPECT(cnt[0] + cnt[1] == 2000);
         EXPECT(std::fabs(cnt[0] / 2000.0 - 0.9) < 0.05);
-        // measure draws its own uniform: outcome 0 about 9 times in 10
+        // measure draws its own uniform: outcome 0 about 9 times in 10 (a
+        // copy carries its source's generator state, so each is reseeded)
         std::size_t zeros = 0;
         for (int i = 0; i < 400; i++) {
             Hip<Fp> c{b};
+            c.seed(1000 + (std::uint64_t)i);
             zeros += c.measure(1) == 0;
         }
         EXPECT(std::fabs(zeros / 400.0 - 0.9) < 0.08);
