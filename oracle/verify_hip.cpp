@@ -119,7 +119,12 @@ int main(int argc, char **argv)
         VerifyFull verify;
         verify.configureState(8);
         verify.configureChecker<qsl::SampleAllChecker>(1000000, 0.99);
-        check_suite(verify, "Qubits<Default,double> vs Qubits<Hip,double>, 8 qubits", 1e-10);
+        // prob sums in another order (FP64 tree vs serial), as the Omp
+        // backend's reduction does: the reference bounds the postselected
+        // states' Fubini-Study distance by 1e-7 for Default vs Omp
+        // (basic-tests.cpp:437-440), since acos near 1 turns a 1e-16 change
+        // of the overlap into about 1e-8
+        check_suite(verify, "Qubits<Default,double> vs Qubits<Hip,double>, 8 qubits", 1e-7);
         check_gates<VerifyFull, qsl::DefaultGateChecker>(verify);
     }
     {
