@@ -216,6 +216,18 @@ int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p);
 int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome);
 int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots,
                       uint64_t *out);
+/* sample_all_noalloc: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): the outcomes of qslhip_sample_all,
+ * with every device buffer in one static array of the library (about
+ * 6.3 MB per device, held until the process ends) instead of the handle's
+ * lazily allocated sampling buffers.  Full-vector handles of at most 2^24
+ * amplitudes (24 qubits), at most 2^18 uniforms per call (QSLHIP_EINVAL
+ * beyond either; QSLHIP_EKIND for number-preserving handles).  It calls no
+ * allocating routine.  All calls on one device share the array: calls that
+ * overlap on one device race. */
+int qslhip_sample_all_noalloc(qslhip_t h, const double *u, uint64_t nshots,
+                              uint64_t *out);
 /* sampleAll2 <- sampleAll2(N) measure.cpp:237-267: counts[x] = number of
  *               uniforms u with cdf(x-1) <= u < cdf(x) (strict merge rule);
  *               uniforms beyond the total mass are not counted.  ncounts

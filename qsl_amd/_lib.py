@@ -84,6 +84,7 @@ SIGNATURES = {
     "qslhip_postselect": (_I, [_vp, _U, _U, C.POINTER(_D)]),
     "qslhip_measure_all": (_I, [_vp, _D, C.POINTER(_u64)]),
     "qslhip_sample_all": (_I, [_vp, _vp, _u64, _vp]),
+    "qslhip_sample_all_noalloc": (_I, [_vp, _vp, _u64, _vp]),
     "qslhip_sample": (_I, [_vp, _U, _vp, _u64, _vp]),
     "qslhip_sample_all2": (_I, [_vp, _vp, _u64, _vp, _u64]),
     "qslhip_norm": (_I, [_vp, C.POINTER(_D)]),

@@ -929,6 +929,15 @@ int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots, uint64_t *ou
     });
 }
 
+int qslhip_sample_all_noalloc(qslhip_t h, const double *u, uint64_t nshots, uint64_t *out)
+{
+    return guarded([&] {
+        require_state_ro(h);
+        QH_REQUIRE((u && out) || nshots == 0, QSLHIP_EINVAL, "null array");
+        device_sample_noalloc(h, u, nshots, out);
+    });
+}
+
 int qslhip_sample_all2(qslhip_t h, const double *u, uint64_t nshots,
                        uint64_t *counts, uint64_t ncounts)
 {

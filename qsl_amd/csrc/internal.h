@@ -168,6 +168,12 @@ void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome,
 double *sample_run_sums(qslhip_state *h);
 void device_sample(qslhip_state *h, const double *u, uint64_t nshots,
                    uint64_t *out, bool strict = false);
+// The same outcomes with every buffer in a static device array (no
+// allocation): full-vector handles of at most kNoAllocMaxAmps amplitudes,
+// at most kNoAllocMaxShots uniforms per call; sampleAll semantics.
+constexpr uint64_t kNoAllocMaxAmps = 1ull << 24;
+constexpr uint64_t kNoAllocMaxShots = 1ull << 18;
+void device_sample_noalloc(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out);
 
 // NP compact backend (np_compact.hip)
 void np_init(qslhip_state *h);

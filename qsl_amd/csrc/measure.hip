@@ -16,6 +16,8 @@
 // A uniform above the total mass gives 0, as in drawSample.  The compact
 // NP layout keeps a stored CDF (build_cdf_any).
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "gate_math.h"
 
@@ -736,6 +738,100 @@ void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *ou
 }
 
 }  // namespace
+
+// ---- sampling without allocation (qslhip_sample_all_noalloc) -------------------
+//
+// The same kernels as sample_impl (sub-chunk sums from the state, one
+// chunk scan, the two-shot wavefront search), with every buffer in one
+// file-scope device array instead of the handle's lazily allocated ones:
+// [chunk prefix: kNoAllocChunks + 1][sub-chunk sums][uniforms][outcomes].
+// Host code never passes `scratch` itself (its name is a host-side
+// stand-in); it takes each device's address once with hipGetSymbolAddress
+// and passes that.  Every call on one device shares the array and overwrites
+// it; nothing in it must start at zero (each kernel writes what the next
+// reads).  Calls that overlap on one device race.
+namespace {
+
+constexpr uint64_t kNoAllocChunks = kNoAllocMaxAmps / kScanChunk;  // 4096: one-block scan
+constexpr uint64_t kNoAllocDoubles =
+    (kNoAllocChunks + 1) + kNoAllocChunks * kSubsPerChunk + 2 * kNoAllocMaxShots;
+static_assert(kNoAllocMaxAmps % kScanChunk == 0 && kNoAllocChunks <= 4096,
+              "the chunk scan of the static sampler runs in one workgroup");
+
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+double *g_scratch_dev[kMaxDevices] = {};  // per-device address of `scratch`
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on device `dev` (taken once per device,
+// with that device current).
+double *scratch_on(int dev)
+{
+    QH_REQUIRE(dev >= 0 && dev < kMaxDevices, QSLHIP_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        int cur = 0;
+        QH_CHECK(hipGetDevice(&cur));
+        if (cur != dev) QH_CHECK(hipSetDevice(dev));
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        QH_CHECK(e);
+        QH_REQUIRE(ptr != nullptr, QSLHIP_EDEVICE, "no device address for the sampling scratch");
+        g_scratch_dev[dev] = static_cast<double *>(ptr);
+    }
+    return g_scratch_dev[dev];
+}
+
+// FP32 handles compare float uniforms, as the reference's Random<float>.
+__global__ void k_round_to_float(double *u, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = (double)(float)u[i];
+}
+
+template <typename T>
+void sample_noalloc_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out)
+{
+    const uint64_t namps = h->namps;
+    const uint64_t nchunks = (namps + kScanChunk - 1) / kScanChunk;
+    double *base = scratch_on(h->device);
+    double *pre = base;
+    double *subs = pre + kNoAllocChunks + 1;
+    double *du = subs + kNoAllocChunks * kSubsPerChunk;
+    uint64_t *dout = reinterpret_cast<uint64_t *>(du + kNoAllocMaxShots);
+    const cplx<T> *st = static_cast<const cplx<T> *>(h->amps);
+    hipLaunchKernelGGL(k_sub_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0, h->stream, st,
+                       namps, subs, pre);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, pre, nchunks);
+    QH_CHECK(hipMemcpyAsync(du, u, nshots * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (sizeof(T) == 4)
+        hipLaunchKernelGGL(k_round_to_float, dim3(grid_for(nshots, kBlock)), dim3(kBlock), 0,
+                           h->stream, du, nshots);
+    hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(nshots, 2 * kWaves)), dim3(kBlock), 0,
+                       h->stream, st, namps, pre, nchunks, subs, du, nshots, 0, dout);
+    QH_CHECK(hipGetLastError());
+    QH_CHECK(hipMemcpyAsync(out, dout, nshots * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            h->stream));
+    QH_CHECK(hipStreamSynchronize(h->stream));
+}
+
+}  // namespace
+
+void device_sample_noalloc(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out)
+{
+    QH_REQUIRE(!h->np, QSLHIP_EKIND, "the allocation-free sampler takes full-vector handles");
+    QH_REQUIRE(h->namps <= kNoAllocMaxAmps, QSLHIP_EINVAL,
+               "the allocation-free sampler takes states of at most 2^24 amplitudes "
+               "(24 qubits); this one has " + std::to_string(h->namps));
+    QH_REQUIRE(nshots <= kNoAllocMaxShots, QSLHIP_EINVAL,
+               "the allocation-free sampler takes at most 2^18 shots per call; got " +
+                   std::to_string(nshots));
+    if (nshots == 0) return;
+    if (h->prec == 8) sample_noalloc_impl<double>(h, u, nshots, out);
+    else sample_noalloc_impl<float>(h, u, nshots, out);
+}
 
 double *sample_run_sums(qslhip_state *h)
 {
