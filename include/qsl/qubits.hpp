@@ -25,6 +25,7 @@ enum class Type {
     Resize,   // reference: resizeable CPU
     Hip,      // this backend: full state vector on one MI355X
     HipNP,    // this backend: compact number-preserving vector on one MI355X
+    HipResize,  // this backend: resizeable full vector on one MI355X
 };
 
 template <Type T = Type::Default, std::floating_point Fp = double>
@@ -34,6 +35,7 @@ class Qubits;
 
 #include "qsl/qubits/hip.hpp"
 #include "qsl/qubits/hip_np.hpp"
+#include "qsl/qubits/hip_resize.hpp"
 #include "qsl/concepts.hpp"
 
 namespace qsl {

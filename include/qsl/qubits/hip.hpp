@@ -17,44 +17,16 @@
 
 namespace qsl {
 
-template <std::floating_point Fp>
-class Qubits<Type::Hip, Fp> : public hipdetail::Base<Fp> {
-    using B = hipdetail::Base<Fp>;
+namespace hipdetail {
 
-    static qslhip_t create(unsigned n)
-    {
-        qslhip_t h = nullptr;
-        hipdetail::check(qslhip_create(n, hipdetail::prec_of<Fp>(), &h));
-        return h;
-    }
+// The gate set, observables and sampleAll2 of the full-vector backends
+// (Qubits<Hip> and Qubits<HipResize>).
+template <std::floating_point Fp>
+class FullVector : public Base<Fp> {
+protected:
+    using Base<Fp>::Base;
 
 public:
-    using Fp_type = Fp;
-    static inline const std::string name =
-        sizeof(Fp) == 8 ? "Qub<hip,double>" : "Qub<hip,float>";
-
-    // default.hpp:114: |0...0> on nqubits
-    Qubits(unsigned nqubits) : B(create(nqubits), nqubits) {}
-    // default.hpp:128: from a state vector (size must be a power of two)
-    Qubits(const std::vector<complex<Fp>> &state)
-        : B(create(checkStateSize(state)), checkStateSize(state))
-    {
-        this->setState(state);
-    }
-    Qubits(const Qubits &o) = default;
-    Qubits(Qubits &&o) noexcept = default;
-
-    // default.cpp:94-105: throws std::logic_error on a qubit-count mismatch
-    void operator=(const Qubits &old) { this->assign(old); }
-
-    // default.cpp:117-122
-    void print(std::ostream &os = std::cout) const
-    {
-        os << "Number of qubits = " << this->nqubits_ << std::endl;
-        for (const auto &a : this->getState()) os << a << std::endl;
-        os << std::endl;
-    }
-
     // ---- one-qubit gates (default.hpp:208-333) --------------------------------
     void rotateX(unsigned targ, Fp angle) { this->g1(QSLHIP_ROT_X, targ, angle); }
     void rotateY(unsigned targ, Fp angle) { this->g1(QSLHIP_ROT_Y, targ, angle); }
@@ -141,6 +113,48 @@ public:
         for (std::size_t i = 0; i < c.size(); i++) m[i] = static_cast<std::size_t>(c[i]);
         return m;
     }
+};
+
+}  // namespace hipdetail
+
+template <std::floating_point Fp>
+class Qubits<Type::Hip, Fp> : public hipdetail::FullVector<Fp> {
+    using B = hipdetail::FullVector<Fp>;
+
+    static qslhip_t create(unsigned n)
+    {
+        qslhip_t h = nullptr;
+        hipdetail::check(qslhip_create(n, hipdetail::prec_of<Fp>(), &h));
+        return h;
+    }
+
+public:
+    using Fp_type = Fp;
+    static inline const std::string name =
+        sizeof(Fp) == 8 ? "Qub<hip,double>" : "Qub<hip,float>";
+
+    // default.hpp:114: |0...0> on nqubits
+    Qubits(unsigned nqubits) : B(create(nqubits), nqubits) {}
+    // default.hpp:128: from a state vector (size must be a power of two)
+    Qubits(const std::vector<complex<Fp>> &state)
+        : B(create(checkStateSize(state)), checkStateSize(state))
+    {
+        this->setState(state);
+    }
+    Qubits(const Qubits &o) = default;
+    Qubits(Qubits &&o) noexcept = default;
+
+    // default.cpp:94-105: throws std::logic_error on a qubit-count mismatch
+    void operator=(const Qubits &old) { this->assign(old); }
+
+    // default.cpp:117-122
+    void print(std::ostream &os = std::cout) const
+    {
+        os << "Number of qubits = " << this->nqubits_ << std::endl;
+        for (const auto &a : this->getState()) os << a << std::endl;
+        os << std::endl;
+    }
+
 };
 
 }  // namespace qsl

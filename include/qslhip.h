@@ -104,6 +104,11 @@ int qslhip_create_np(unsigned nqubits, unsigned nones, int prec, qslhip_t *out);
 int qslhip_destroy(qslhip_t h);
 int qslhip_clone(qslhip_t src, qslhip_t *out);
 int qslhip_assign(qslhip_t dst, qslhip_t src);
+/* qslhip_create_resize <- Qubits<Resize,Fp>(unsigned nqubits)
+ *                     src/qubits/resize/resize.cpp:39-46: a full-vector
+ *                     handle whose qubits can be added and removed
+ *                     (qslhip_add_qubit, qslhip_measure_out, ...). */
+int qslhip_create_resize(unsigned nqubits, int prec, qslhip_t *out);
 /* nones is 0 for a full-vector handle; namps is the device amplitude count. */
 int qslhip_info(qslhip_t h, unsigned *nqubits, unsigned *nones, int *prec,
                 uint64_t *namps);
@@ -216,6 +221,32 @@ int qslhip_postselect(qslhip_t h, unsigned targ, unsigned outcome, double *p);
 int qslhip_measure_all(qslhip_t h, double u, uint64_t *outcome);
 int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots,
                       uint64_t *out);
+/* ---- resizeable handles (src/qubits/resize/) ----------------------------------
+ * Only on handles made by qslhip_create_resize (QSLHIP_EKIND otherwise).
+ * resize_set_state <- Resize::setState resize.cpp:70-81: any power-of-two
+ *               count (QSLHIP_ESIZE otherwise); the qubit count follows.
+ * add_qubit  <- addQubit(targ) resize.cpp:107-147: a new qubit in |0> at
+ *               index targ <= n (targ == n: appendQubit, :101-105); qubits
+ *               k >= targ move to k + 1.  When 2^(n+1) amplitudes exceed
+ *               the allocation it doubles (out of place), else the move
+ *               runs in place.
+ * collapse_out <- collapseOut measure.cpp:62-94 (private in the reference):
+ *               keep qubit targ == outcome, times factor, and remove targ.
+ * measure_out <- measureOut(targ) measure.cpp:157-190 (measure's rule with
+ *               the injected uniform u, prob0 and factor in FP64).
+ * postselect_out <- postselectOut(targ, outcome) measure.cpp:235-250.
+ * reallocate <- reallocateState() resize.cpp:93-98: the allocation trimmed
+ *               to 2^n amplitudes.
+ * capacity:   amplitudes the allocation holds (>= 2^n).
+ * Removing the last qubit is refused (QSLHIP_EINVAL). */
+int qslhip_resize_set_state(qslhip_t h, const void *host, uint64_t count);
+int qslhip_add_qubit(qslhip_t h, unsigned targ);
+int qslhip_collapse_out(qslhip_t h, unsigned targ, unsigned outcome, double factor);
+int qslhip_measure_out(qslhip_t h, unsigned targ, double u, int *outcome);
+int qslhip_postselect_out(qslhip_t h, unsigned targ, unsigned outcome, double *p);
+int qslhip_reallocate(qslhip_t h);
+int qslhip_capacity(qslhip_t h, uint64_t *cap);
+
 /* sample_all_noalloc: no counterpart in the reference.  For callers that
  * cannot allocate device memory (code under HIP graph capture, serving
  * loops, harnesses that forbid hipMalloc): the outcomes of qslhip_sample_all,

@@ -70,6 +70,10 @@ class Oracle:
             f("orc_sample_all2", None, _vp, U, _vp, S, _vp)
             f("orc_sample", None, _vp, U, U, _vp, S, _vp)
             f("orc_norm", R, _vp, u64)
+            f("orc_add_qubit", None, _vp, U, U)
+            f("orc_collapse_out", None, _vp, U, U, U, R)
+            f("orc_measure_out", I, _vp, U, U, R)
+            f("orc_postselect_out", R, _vp, U, U, U)
             f("orc_normalise", R, _vp, u64)
             f("orc_fubini_study", R, _vp, _vp, u64)
             f("orc_check_state_np", I, _vp, u64, C.POINTER(C.c_uint))
@@ -212,6 +216,29 @@ class Oracle:
 
     def norm(self, st):
         return self._fn("orc_norm", st)(_ptr(st), st.size)
+
+    # -- Resize semantics (src/qubits/resize/): each returns the new vector --
+    def add_qubit(self, st, t):
+        n = self.nqubits(st)
+        buf = np.zeros(2 * st.size, st.dtype)
+        buf[:st.size] = st
+        self._fn("orc_add_qubit", st)(_ptr(buf), n, t)
+        return buf
+
+    def collapse_out(self, st, t, o, factor):
+        buf = np.ascontiguousarray(st).copy()
+        self._fn("orc_collapse_out", st)(_ptr(buf), self.nqubits(st), t, o, factor)
+        return buf[:st.size // 2].copy()
+
+    def measure_out(self, st, t, u):
+        buf = np.ascontiguousarray(st).copy()
+        out = self._fn("orc_measure_out", st)(_ptr(buf), self.nqubits(st), t, u)
+        return out, buf[:st.size // 2].copy()
+
+    def postselect_out(self, st, t, o):
+        buf = np.ascontiguousarray(st).copy()
+        p = self._fn("orc_postselect_out", st)(_ptr(buf), self.nqubits(st), t, o)
+        return p, buf[:st.size // 2].copy()
 
     def normalise(self, st):
         return self._fn("orc_normalise", st)(_ptr(st), st.size)

@@ -75,6 +75,11 @@ typedef struct {
     void orc_sample_##S(const R *st, unsigned n, unsigned t, const R *u,        \
                         size_t nshots, uint64_t *counts2);                      \
     R orc_norm_##S(const R *st, uint64_t dim);                                  \
+    void orc_add_qubit_##S(R *st, unsigned n, unsigned targ);                   \
+    void orc_collapse_out_##S(R *st, unsigned n, unsigned targ,                 \
+                              unsigned outcome, R factor);                      \
+    int orc_measure_out_##S(R *st, unsigned n, unsigned targ, R u);             \
+    R orc_postselect_out_##S(R *st, unsigned n, unsigned targ, unsigned outcome); \
     R orc_normalise_##S(R *st, uint64_t dim);                                   \
     R orc_fubini_study_##S(const R *v, const R *w, uint64_t dim);               \
     int orc_check_state_np_##S(const R *st, uint64_t dim, unsigned *nones);     \

@@ -3,12 +3,12 @@
 // (include/qsl/verify/checkers.hpp: ProbChecker :224, MeasureChecker :305,
 // SampleChecker :462, SampleAllChecker :589, DefaultGateChecker :945,
 // NPGateChecker :1082, PostselectChecker :1178), instantiated with the MI355X
-// backend against the reference's Default and NP backends.
+// backends against the reference's Default, NP and Resize backends.
 //
 // TEST INFRASTRUCTURE.  oracle/Makefile (target `verify`) compiles this file
 // against the reference's headers read in place under /root/reference, with
 // one of them, qsl/qubits.hpp, patched in the build folder the way the
-// integration guide describes (the Hip and HipNP tags added to the Type enum,
+// integration guide describes (the Hip, HipNP and HipResize tags added to the Type enum,
 // and the backend's headers included after the reference's own); it links
 // the reference library built from qsl's sources (oracle/_ref/libqslref.so)
 // and libqslhip.so.  The binary goes to oracle/_ref/verify_hip.
@@ -116,6 +116,12 @@ using VerifyNP = qsl::Verify<NP, HipNP, qsl::NPStateGen<double>, qsl::MeasureChe
 // the number-preserving gate set of the full-vector backend against NP
 using VerifyFullNP = qsl::Verify<NP, Hip, qsl::NPStateGen<double>, qsl::ProbChecker,
                                  qsl::NPGateChecker>;
+// the resizeable pair, as the reference's Default-vs-Resize case
+using Resize = qsl::Qubits<qsl::Type::Resize, double>;
+using HipResize = qsl::Qubits<qsl::Type::HipResize, double>;
+using VerifyResize = qsl::Verify<Resize, HipResize, qsl::DefaultStateGen<double>,
+                                 qsl::MeasureChecker, qsl::PostselectChecker, qsl::SampleChecker,
+                                 qsl::SampleAllChecker, qsl::ProbChecker, qsl::DefaultGateChecker>;
 
 }  // namespace
 
@@ -124,7 +130,9 @@ int main(int argc, char **argv)
     if (argc > 1 && std::strcmp(argv[1], "--compile") == 0) {
         static_assert(qsl::Simulator<Hip> && qsl::HasAllGates<Hip>);
         static_assert(qsl::Simulator<HipNP> && qsl::NPSimulator<HipNP>);
-        std::printf("verify harness instantiated with Qubits<Hip> and Qubits<HipNP>\n");
+        static_assert(qsl::Simulator<HipResize> && qsl::HasAllGates<HipResize>);
+        std::printf("verify harness instantiated with Qubits<Hip>, Qubits<HipNP> and "
+                    "Qubits<HipResize>\n");
         return 0;
     }
     {
@@ -146,6 +154,13 @@ int main(int argc, char **argv)
         // the same summation-order bound as above
         check_suite(verify, "Qubits<NP,double> vs Qubits<HipNP,double>, 8 qubits, 5 ones", 1e-7);
         check_gates<VerifyNP, qsl::NPGateChecker>(verify);
+    }
+    {
+        VerifyResize verify;
+        verify.configureState(8);
+        configure_statistics(verify);
+        check_suite(verify, "Qubits<Resize,double> vs Qubits<HipResize,double>, 8 qubits", 1e-7);
+        check_gates<VerifyResize, qsl::DefaultGateChecker>(verify);
     }
     {
         VerifyFullNP verify;

@@ -9,5 +9,5 @@ from ._lib import (QslHipError, SizeError, NotNPError, uniforms, device_count,  
                    OP_DTYPE, MODE_PER_GATE, MODE_FUSED, MODE_DENSE, MODE_REORDER, LIB_PATH,
                    HEADER_PATH)
 from . import _lib as abi  # noqa: F401
-from .qubits import Qubits, QubitsNP, Batch  # noqa: F401
+from .qubits import Qubits, QubitsNP, QubitsResize, Batch  # noqa: F401
 from ._lib import PAULI_DTYPE  # noqa: F401

@@ -85,6 +85,14 @@ SIGNATURES = {
     "qslhip_measure_all": (_I, [_vp, _D, C.POINTER(_u64)]),
     "qslhip_sample_all": (_I, [_vp, _vp, _u64, _vp]),
     "qslhip_sample_all_noalloc": (_I, [_vp, _vp, _u64, _vp]),
+    "qslhip_create_resize": (_I, [_U, _I, C.POINTER(_vp)]),
+    "qslhip_resize_set_state": (_I, [_vp, _vp, _u64]),
+    "qslhip_add_qubit": (_I, [_vp, _U]),
+    "qslhip_collapse_out": (_I, [_vp, _U, _U, _D]),
+    "qslhip_measure_out": (_I, [_vp, _U, _D, C.POINTER(_I)]),
+    "qslhip_postselect_out": (_I, [_vp, _U, _U, C.POINTER(_D)]),
+    "qslhip_reallocate": (_I, [_vp]),
+    "qslhip_capacity": (_I, [_vp, C.POINTER(_u64)]),
     "qslhip_sample": (_I, [_vp, _U, _vp, _u64, _vp]),
     "qslhip_sample_all2": (_I, [_vp, _vp, _u64, _vp, _u64]),
     "qslhip_norm": (_I, [_vp, C.POINTER(_D)]),

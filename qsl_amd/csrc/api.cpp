@@ -81,6 +81,7 @@ qslhip_state *alloc_handle(unsigned n, int prec, bool np, unsigned k)
     h->np = np;
     h->k = k;
     h->namps = np ? choose_u64(n, k) : (1ull << n);
+    h->cap = h->namps;
     try {
         QH_CHECK(hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
         h->stream = h->own_stream;
@@ -265,6 +266,22 @@ int qslhip_create(unsigned nqubits, int prec, qslhip_t *out)
     });
 }
 
+int qslhip_create_resize(unsigned nqubits, int prec, qslhip_t *out)
+{
+    return guarded([&] {
+        QH_REQUIRE(out, QSLHIP_EINVAL, "null output handle");
+        *out = nullptr;
+        QH_REQUIRE(prec == 4 || prec == 8, QSLHIP_EINVAL, "prec must be 4 or 8");
+        QH_REQUIRE(nqubits >= 1 && nqubits <= kMaxQubits, QSLHIP_EINVAL,
+                   "nqubits must be in [1, 40]");
+        qslhip_t h = alloc_handle(nqubits, prec, false, 0);
+        h->resizable = true;
+        reset_impl(h);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *out = h;
+    });
+}
+
 int qslhip_create_np(unsigned nqubits, unsigned nones, int prec, qslhip_t *out)
 {
     return guarded([&] {
@@ -314,6 +331,7 @@ int qslhip_clone(qslhip_t src, qslhip_t *out)
         QH_REQUIRE(out, QSLHIP_EINVAL, "null output handle");
         *out = nullptr;
         qslhip_t h = alloc_handle(src->n, src->prec, src->np, src->k);
+        h->resizable = src->resizable;
         QH_CHECK(hipStreamSynchronize(src->stream));
         QH_CHECK(hipMemcpyAsync(h->amps, src->amps, src->namps * src->amp_bytes(),
                                 hipMemcpyDeviceToDevice, h->stream));
@@ -926,6 +944,148 @@ int qslhip_sample_all(qslhip_t h, const double *u, uint64_t nshots, uint64_t *ou
         }
         if (h->np) np_sample(h, src, nshots, out);
         else device_sample(h, src, nshots, out);
+    });
+}
+
+// ---- resizeable handles (src/qubits/resize/) ------------------------------------
+
+static void require_resizable(qslhip_t h)
+{
+    QH_REQUIRE(h->resizable, QSLHIP_EKIND,
+               "qubits are added and removed only on a resizeable handle (qslhip_create_resize)");
+}
+
+int qslhip_capacity(qslhip_t h, uint64_t *cap)
+{
+    return guarded([&] {
+        require_handle(h);
+        QH_REQUIRE(cap, QSLHIP_EINVAL, "null output");
+        *cap = h->cap;
+    });
+}
+
+int qslhip_resize_set_state(qslhip_t h, const void *host, uint64_t count)
+{
+    return guarded([&] {
+        require_handle(h);
+        require_resizable(h);
+        QH_REQUIRE(host, QSLHIP_EINVAL, "null host pointer");
+        // resize.cpp:70-81 (checkStateSize, quantum.cpp:108-131)
+        QH_REQUIRE(count >= 2 && (count & (count - 1)) == 0, QSLHIP_ESIZE,
+                   "State vector size is not a power of two");
+        QH_REQUIRE(count <= (1ull << kMaxQubits), QSLHIP_EINVAL, "state too large");
+        if (count > h->cap) {
+            void *p = nullptr;
+            hipError_t e = hipMalloc(&p, count * h->amp_bytes());
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(QSLHIP_ENOMEM, "hipMalloc of " +
+                                               std::to_string(count * h->amp_bytes()) +
+                                               " bytes for the state failed");
+            }
+            QH_CHECK(hipStreamSynchronize(h->stream));
+            QH_CHECK(hipFree(h->amps));
+            h->amps = p;
+            h->cap = count;
+        }
+        h->n = (unsigned)__builtin_ctzll(count);
+        h->namps = count;
+        h->pending_basis = qslhip_state::kNoBasis;
+        h->sums_valid = false;
+        QH_CHECK(hipMemcpyAsync(h->amps, host, count * h->amp_bytes(), hipMemcpyHostToDevice,
+                                h->stream));
+        QH_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int qslhip_add_qubit(qslhip_t h, unsigned targ)
+{
+    return guarded([&] {
+        require_handle(h);
+        require_resizable(h);
+        // resize.cpp:107-147; targ == n appends (appendQubit, :101-105)
+        QH_REQUIRE(targ <= h->n, QSLHIP_EINVAL, "qubit index out of range");
+        QH_REQUIRE(h->n + 1 <= kMaxQubits, QSLHIP_EINVAL, "too many qubits");
+        require_state(h);
+        resize_add_qubit(h, targ);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int qslhip_collapse_out(qslhip_t h, unsigned targ, unsigned outcome, double factor)
+{
+    return guarded([&] {
+        require_handle(h);
+        require_resizable(h);
+        QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
+        QH_REQUIRE(h->n >= 2, QSLHIP_EINVAL, "cannot remove the last qubit");
+        require_state(h);
+        // measure.cpp:62-94
+        resize_collapse_out(h, targ, outcome, factor);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int qslhip_measure_out(qslhip_t h, unsigned targ, double u, int *outcome)
+{
+    return guarded([&] {
+        require_handle(h);
+        require_resizable(h);
+        QH_REQUIRE(outcome, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(targ < h->n, QSLHIP_EINVAL, "qubit index out of range");
+        QH_REQUIRE(h->n >= 2, QSLHIP_EINVAL, "cannot remove the last qubit");
+        require_state(h);
+        // measure.cpp:157-190, with prob0 and the factor in FP64 as measure
+        double prob0 = device_prob(h, targ, 0);
+        double uu = as_fp(h, u);
+        int o;
+        double factor;
+        if (uu < prob0) {
+            o = 0;
+            factor = 1.0 / std::sqrt(prob0);
+        } else {
+            o = 1;
+            factor = 1.0 / std::sqrt(1.0 - prob0);
+        }
+        resize_collapse_out(h, targ, (unsigned)o, factor);
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *outcome = o;
+    });
+}
+
+int qslhip_postselect_out(qslhip_t h, unsigned targ, unsigned outcome, double *p)
+{
+    return guarded([&] {
+        require_handle(h);
+        require_resizable(h);
+        QH_REQUIRE(p, QSLHIP_EINVAL, "null output");
+        QH_REQUIRE(targ < h->n && outcome <= 1, QSLHIP_EINVAL, "bad qubit or outcome");
+        QH_REQUIRE(h->n >= 2, QSLHIP_EINVAL, "cannot remove the last qubit");
+        require_state(h);
+        // measure.cpp:235-250
+        double pr = device_prob(h, targ, outcome);
+        resize_collapse_out(h, targ, outcome, 1.0 / std::sqrt(pr));
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        *p = pr;
+    });
+}
+
+int qslhip_reallocate(qslhip_t h)
+{
+    return guarded([&] {
+        require_handle(h);
+        require_resizable(h);
+        // resize.cpp:93-98: trim the allocation to the state
+        if (h->cap == h->namps) return;
+        require_state_ro(h);
+        void *p = nullptr;
+        QH_CHECK(hipMalloc(&p, h->namps * h->amp_bytes()));
+        QH_CHECK(hipMemcpyAsync(p, h->amps, h->namps * h->amp_bytes(), hipMemcpyDeviceToDevice,
+                                h->stream));
+        QH_CHECK(hipStreamSynchronize(h->stream));
+        QH_CHECK(hipFree(h->amps));
+        h->amps = p;
+        h->cap = h->namps;
     });
 }
 

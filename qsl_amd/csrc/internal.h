@@ -106,6 +106,8 @@ struct qslhip_state {
     bool np = false;       // compact number-preserving layout
     int prec = 8;          // 8 double, 4 float
     uint64_t namps = 0;    // device amplitudes
+    uint64_t cap = 0;      // amplitudes the allocation holds (>= namps)
+    bool resizable = false;  // qubits may be added and removed (Resize backend)
     void *amps = nullptr;  // device state
     // A basis state that reset/setBasis recorded but has not written yet
     // (kNoBasis: `amps` holds the state).  Every reader writes it first; a
@@ -174,6 +176,11 @@ void device_sample(qslhip_state *h, const double *u, uint64_t nshots,
 constexpr uint64_t kNoAllocMaxAmps = 1ull << 24;
 constexpr uint64_t kNoAllocMaxShots = 1ull << 18;
 void device_sample_noalloc(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *out);
+
+// resizeable backend (resize.hip): addQubit and collapseOut, in place
+// where the allocation allows (the state's basis record must be written)
+void resize_add_qubit(qslhip_state *h, unsigned targ);
+void resize_collapse_out(qslhip_state *h, unsigned targ, unsigned outcome, double factor);
 
 // NP compact backend (np_compact.hip)
 void np_init(qslhip_state *h);

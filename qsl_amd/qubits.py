@@ -239,6 +239,66 @@ class Qubits:
         return [int(counts[0]), int(counts[1])]
 
 
+class QubitsResize(Qubits):
+    """Resizeable backend (include/qsl/qubits/resize.hpp): qubits are added
+    and measured or postselected out on the device; the allocation doubles
+    when a qubit no longer fits and shrinks only in reallocateState."""
+
+    def __init__(self, nqubits_or_state, dtype=np.complex128, seed=0):
+        super().__init__(nqubits_or_state, dtype=dtype, seed=seed)
+        self.name = "Qub<hipresize,%s>" % ("double" if self.prec == 8 else "float")
+
+    def _create(self, lib, n, h):
+        return lib.qslhip_create_resize(n, self.prec, C.byref(h))
+
+    def _refresh(self):
+        n = C.c_uint()
+        check(L.lib().qslhip_info(self._h, C.byref(n), None, None, None))
+        self.nqubits = n.value
+
+    def setState(self, st):
+        """resize.cpp:70-81: any power-of-two size; the qubit count follows."""
+        st = np.ascontiguousarray(st, dtype=self.dtype)
+        if st.size < 2 or st.size & (st.size - 1):
+            raise L.SizeError(L.ESIZE, "State vector size is not a power of two")
+        check(L.lib().qslhip_resize_set_state(self._h, ptr(st), st.size))
+        self._refresh()
+
+    def addQubit(self, t):
+        check(L.lib().qslhip_add_qubit(self._h, t))
+        self._refresh()
+
+    def appendQubit(self):
+        self.addQubit(self.nqubits)
+
+    def reallocateState(self):
+        check(L.lib().qslhip_reallocate(self._h))
+
+    def collapseOut(self, t, outcome, factor):
+        check(L.lib().qslhip_collapse_out(self._h, t, outcome, factor))
+        self._refresh()
+
+    def measureOut(self, t, u=None):
+        if u is None:
+            u = float(self._draw(1)[0])
+        o = C.c_int()
+        check(L.lib().qslhip_measure_out(self._h, t, u, C.byref(o)))
+        self._refresh()
+        return o.value
+
+    def postselectOut(self, t, outcome):
+        p = C.c_double()
+        check(L.lib().qslhip_postselect_out(self._h, t, outcome, C.byref(p)))
+        self._refresh()
+        return p.value
+
+    def capacity(self):
+        """Amplitudes the device allocation holds (>= 2^nqubits)."""
+        v = C.c_uint64()
+        check(L.lib().qslhip_capacity(self._h, C.byref(v)))
+        return v.value
+
+
 class QubitsNP(Qubits):
     """Number-preserving backend: only the C(n, nones) amplitudes of weight
     nones live on the device (compact colex-rank layout)."""

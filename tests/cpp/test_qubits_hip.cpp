@@ -31,6 +31,9 @@ static_assert(qsl::Simulator<qsl::Qubits<qsl::Type::HipNP, double>>);
 static_assert(qsl::NPSimulator<qsl::Qubits<qsl::Type::HipNP, double>>);
 static_assert(qsl::NPSimulator<qsl::Qubits<qsl::Type::HipNP, float>>);
 static_assert(!qsl::HasNonNPGates<qsl::Qubits<qsl::Type::HipNP, double>>);
+static_assert(qsl::Simulator<qsl::Qubits<qsl::Type::HipResize, double>>);
+static_assert(qsl::HasAllGates<qsl::Qubits<qsl::Type::HipResize, float>>);
+static_assert(qsl::HasMeasurement<qsl::Qubits<qsl::Type::HipResize, double>>);
 
 // ---- a minimal test runner ---------------------------------------------------
 static int g_fail = 0, g_checks = 0;
@@ -402,6 +405,127 @@ static void np_gate_test()
 }
 TEST(NPGatesDouble) { np_gate_test<double>(); }
 TEST(NPGatesFloat) { np_gate_test<float>(); }
+
+// ---- resizeable backend (src/gtest/resize-test.cpp) -------------------------------
+template <typename Fp>
+using HipRz = qsl::Qubits<qsl::Type::HipResize, Fp>;
+
+template <typename Fp>
+struct OrcRz;
+template <>
+struct OrcRz<double> {
+    static void add(double *s, unsigned n, unsigned t) { orc_add_qubit_d(s, n, t); }
+    static double post(double *s, unsigned n, unsigned t, unsigned o)
+    {
+        return orc_postselect_out_d(s, n, t, o);
+    }
+    static void out(double *s, unsigned n, unsigned t, unsigned o, double f)
+    {
+        orc_collapse_out_d(s, n, t, o, f);
+    }
+    static double prob(const double *s, unsigned n, unsigned t, unsigned o) { return orc_prob_d(s, n, t, o); }
+};
+template <>
+struct OrcRz<float> {
+    static void add(float *s, unsigned n, unsigned t) { orc_add_qubit_f(s, n, t); }
+    static float post(float *s, unsigned n, unsigned t, unsigned o)
+    {
+        return orc_postselect_out_f(s, n, t, o);
+    }
+    static void out(float *s, unsigned n, unsigned t, unsigned o, float f)
+    {
+        orc_collapse_out_f(s, n, t, o, f);
+    }
+    static float prob(const float *s, unsigned n, unsigned t, unsigned o) { return orc_prob_f(s, n, t, o); }
+};
+
+template <typename Fp>
+static void resize_test()
+{
+    using Sim = HipRz<Fp>;
+    const double tol = sizeof(Fp) == 8 ? 1e-12 : 1e-5;
+    // constructors and assignment (resize-test.cpp:32-76)
+    const unsigned n = 5;
+    const auto state = qsl::makeRandomState<Fp>(n, 21);
+    Sim q{n};
+    q.setState(state);
+    Sim q_copy{q};
+    EXPECT(q_copy.getNumQubits() == n && same_bits(q_copy.getState(), state));
+    Sim q_new{n + 2};
+    q_new = q;  // any size
+    EXPECT(q_new.getNumQubits() == n && same_bits(q_new.getState(), state));
+    EXPECT(Sim::name == (sizeof(Fp) == 8 ? "Qub<hipresize,double>" : "Qub<hipresize,float>"));
+    // setState of another size (resize-test.cpp:79-100)
+    Sim u{3};
+    u.setState(qsl::makeRandomState<Fp>(4, 2));
+    EXPECT(u.getNumQubits() == 4);
+    u.setBasisState(4);
+    EXPECT(u.getState()[4].real == 1);
+    // appendQubit (resize-test.cpp:260-282): the old vector then zeros
+    {
+        Sim a{state};
+        a.appendQubit();
+        auto v = a.getState();
+        EXPECT(a.getNumQubits() == n + 1 && v.size() == state.size() * 2);
+        bool ok = true;
+        for (std::size_t i = 0; i < v.size(); i++) {
+            const auto w = i < state.size() ? state[i] : qsl::complex<Fp>{0, 0};
+            ok = ok && v[i].real == w.real && v[i].imag == w.imag;
+        }
+        EXPECT(ok);
+    }
+    // addQubit at every index, against the oracle (bit for bit)
+    for (unsigned t = 0; t <= n; t++) {
+        Sim a{state};
+        a.addQubit(t);
+        std::vector<qsl::complex<Fp>> want(state.size() * 2);
+        std::copy(state.begin(), state.end(), want.begin());
+        OrcRz<Fp>::add(raw(want), n, t);
+        EXPECT(same_bits(a.getState(), want));
+    }
+    // postselectOut (resize-test.cpp:193-258)
+    for (unsigned t = 0; t < n; t++)
+        for (unsigned o = 0; o < 2; o++) {
+            Sim a{state};
+            Fp p = a.postselectOut(t, o);
+            auto want = state;
+            Fp wp = OrcRz<Fp>::post(raw(want), n, t, o);
+            want.resize(state.size() / 2);
+            EXPECT(std::fabs(p - wp) < tol && a.getNumQubits() == n - 1);
+            EXPECT(max_diff(a.getState(), want) < 10 * tol);
+        }
+    // measureOut (resize-test.cpp:103-190): outcome frequencies and the
+    // collapsed state for every outcome drawn
+    for (unsigned t = 0; t < n; t++) {
+        std::size_t ones = 0;
+        const int samples = 300;
+        for (int k = 0; k < samples; k++) {
+            Sim a{state};
+            a.seed(77 + 1000 * t + (std::uint64_t)k);
+            int o = a.measureOut(t);
+            ones += o;
+            auto want = state;
+            const double p0 = OrcRz<Fp>::prob(raw(want), n, t, 0);
+            const double f = o == 0 ? 1 / std::sqrt(p0) : 1 / std::sqrt(1 - p0);
+            OrcRz<Fp>::out(raw(want), n, t, (unsigned)o, (Fp)f);
+            want.resize(state.size() / 2);
+            EXPECT(max_diff(a.getState(), want) < 100 * tol);
+        }
+        const double p1 = OrcRz<Fp>::prob(raw(const_cast<std::vector<qsl::complex<Fp>> &>(state)), n, t, 1);
+        EXPECT(std::fabs((double)ones / samples - p1) < 0.12);  // 4 sigma at 300
+    }
+    // remove then add again inside the allocation, then trim it
+    {
+        Sim a{state};
+        a.postselectOut(2, 0);
+        a.addQubit(2);
+        a.reallocateState();
+        EXPECT(a.getNumQubits() == n && a.prob(2, 1) == 0);
+    }
+    EXPECT_THROW(Sim{1}.postselectOut(0, 0), std::out_of_range);
+}
+TEST(ResizeDouble) { resize_test<double>(); }
+TEST(ResizeFloat) { resize_test<float>(); }
 
 // ---- expectation values, adjoint gradients, batches (no reference
 // counterpart; pinned by the mathematics: parameter shift is exact for
