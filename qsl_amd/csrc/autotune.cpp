@@ -112,6 +112,8 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "reg_min_qubits") *value = t.reg_min_qubits;
         else if (k == "jit") *value = t.jit;
         else if (k == "jit_waves") *value = t.jit_waves;
+        else if (k == "stream_min_qubits") *value = t.stream_min_qubits;
+        else if (k == "lane_gates") *value = t.lane_gates;
         else if (k == "np_bonds") *value = t.np_bonds;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
@@ -162,6 +164,13 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "np_bonds") {
             QH_REQUIRE(value >= 1 && value <= 4, QSLHIP_EINVAL, "np_bonds must be in [1, 4]");
             t.np_bonds = value;
+        } else if (k == "lane_gates") {
+            QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "lane_gates must be 0 or 1");
+            t.lane_gates = value;
+        } else if (k == "stream_min_qubits") {
+            QH_REQUIRE(value >= 0 && value <= 64, QSLHIP_EINVAL,
+                       "stream_min_qubits must be in [0, 64]");
+            t.stream_min_qubits = value;
         } else if (k == "jit_waves") {
             QH_REQUIRE(value >= 0 && value <= 8, QSLHIP_EINVAL, "jit_waves must be in [0, 8]");
             t.jit_waves = value;

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(cplx<T> *__restrict__ st,
 
 // The same pass with one op passed by value (the per-gate path for low
 // targets): no op table to stage.
-template <typename T>
+template <typename T, bool S>
 __global__ __launch_bounds__(kBlock) void k_tile1(cplx<T> *__restrict__ st, unsigned n,
                                                   int K, int crun, uint64_t tile_mask,
                                                   TileOp op)
@@ -79,12 +79,12 @@ __global__ __launch_bounds__(kBlock) void k_tile1(cplx<T> *__restrict__ st, unsi
     const uint32_t nt = 1u << K;
     const uint32_t lowm = (1u << crun) - 1;
     for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        tile[e] = st[gbase | (e & lowm) | hoff[e >> crun]];
+        tile[e] = amp_ld<S>(st + (gbase | (e & lowm) | hoff[e >> crun]));
     __syncthreads();
     tile_op<T>(tile, K, gbase, op);
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        st[gbase | (e & lowm) | hoff[e >> crun]] = tile[e];
+        amp_st<S>(st + (gbase | (e & lowm) | hoff[e >> crun]), tile[e]);
 }
 
 template <typename T>
@@ -102,9 +102,18 @@ void run(qslhip_state *h, const std::vector<TilePass> &passes, const TileOp *dop
         QH_REQUIRE(shmem <= 64 * 1024, QSLHIP_EINVAL, "tile pass needs more than 64 KiB of LDS");
         uint64_t groups = 1ull << (h->n - p.logt);
         if (single) {
-            hipLaunchKernelGGL(k_tile1<T>, dim3((unsigned)groups), dim3(kBlock), shmem,
-                               h->stream, static_cast<cplx<T> *>(amps), h->n, p.logt, crun,
-                               p.tile_mask, *single);
+            // whole contiguous runs of at least 128 bytes: streaming accesses
+            // for states far larger than the caches
+            const bool stream = (int)h->n >= tuning().stream_min_qubits &&
+                                (sizeof(cplx<T>) << crun) >= 128;
+            if (stream)
+                hipLaunchKernelGGL((k_tile1<T, true>), dim3((unsigned)groups), dim3(kBlock),
+                                   shmem, h->stream, static_cast<cplx<T> *>(amps), h->n,
+                                   p.logt, crun, p.tile_mask, *single);
+            else
+                hipLaunchKernelGGL((k_tile1<T, false>), dim3((unsigned)groups), dim3(kBlock),
+                                   shmem, h->stream, static_cast<cplx<T> *>(amps), h->n,
+                                   p.logt, crun, p.tile_mask, *single);
             QH_CHECK(hipGetLastError());
             continue;
         }

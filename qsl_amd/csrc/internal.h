@@ -218,7 +218,8 @@ bool is_np_kind(int kind);
 
 // Tunables picked by the autotuner (autotune.cpp), with safe defaults.
 struct Tuning {
-    int logt_f64 = 11, low_f64 = 3;
+    int logt_f64 = 11, low_f64 = 4;  // 4: 256-byte FP64 runs per load (12% faster
+                                     // headline step than 3 at 30 qubits)
     int logt_f32 = 11, low_f32 = 3;
     int lds_max_target_f64 = 3;  // per-gate path: targets below use a tile
     int lds_max_target_f32 = 4;
@@ -232,6 +233,11 @@ struct Tuning {
                                  // 1 once a pass structure repeats, 2 always
     int np_bonds = 1;            // NP: swap-like gates on disjoint pairs per pass (1: one
                                  // each; grouping measured slower for config 3)
+    int lane_gates = 1;          // one-qubit gates on targets 0..5 by cross-lane
+                                 // exchange (0: one-op LDS tiles / streaming pairs)
+    int stream_min_qubits = 26;  // states of at least this many qubits (1 GiB FP64)
+                                 // use streaming loads and stores in the gate
+                                 // and reduction kernels; smaller ones stay cached
     int jit_waves = 0;           // compiled passes: register budget for this many
                                  // waves per SIMD (0: the compiler's choice)
 };

@@ -54,7 +54,7 @@ __device__ __forceinline__ double sq(const cplx<T> &a)
 }
 
 // Sum of |a|^2 over amplitudes with bit t == outcome (t < 0: all).
-template <typename T>
+template <typename T, bool S>
 __global__ __launch_bounds__(kBlock) void k_prob_partial(const cplx<T> *__restrict__ st,
                                                          uint64_t nitems, int t,
                                                          unsigned outcome,
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_prob_partial(const cplx<T> *__restri
     const uint64_t ob = (t >= 0) ? ((uint64_t)outcome << t) : 0;
     for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < nitems; f += stride) {
         uint64_t i = (t >= 0) ? (ins0(f, (unsigned)t) | ob) : f;
-        acc += sq(st[i]);
+        acc += sq(amp_ld<S>(st + i));
     }
     double r = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
@@ -84,14 +84,14 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
 
 // collapse (measure.cpp:31-57): kept half times factor, the other zeroed.
 // The factor is FP64; a float state is scaled in FP64 and rounded once.
-template <typename T>
+template <typename T, bool S>
 __global__ __launch_bounds__(kBlock) void k_collapse(cplx<T> *__restrict__ st,
                                                      uint64_t namps, unsigned t,
                                                      unsigned outcome, double factor)
 {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < namps; i += stride) {
-        cplx<T> a = st[i];
+        cplx<T> a = amp_ld<S>(st + i);
         if (((i >> t) & 1ull) == outcome) {
             if constexpr (sizeof(T) == 8) {
                 a.re *= factor;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_collapse(cplx<T> *__restrict__ st,
             a.re = 0;
             a.im = 0;
         }
-        st[i] = a;
+        amp_st<S>(st + i, a);
     }
 }
 
@@ -258,7 +258,7 @@ constexpr int kSubsPerWave = kSubsPerChunk / kWaves;  // 16
 
 // Sub-chunk sums (one wave reduction each) and chunk sums (their wave
 // totals added in sub order, then the four waves in order).
-template <typename T>
+template <typename T, bool S>
 __global__ __launch_bounds__(kBlock) void k_sub_sums(const cplx<T> *__restrict__ st,
                                                      uint64_t namps, double *__restrict__ subs,
                                                      double *__restrict__ sums)
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_sub_sums(const cplx<T> *__restrict__
 #pragma unroll
     for (int j = 0; j < kSubsPerWave; j++) {
         const uint64_t i = a0 + (uint64_t)j * kSub + lane;
-        p[j] = i < namps ? sq(st[i]) : 0.0;
+        p[j] = i < namps ? sq(amp_ld<S>(st + i)) : 0.0;
     }
     double tot = 0, mine = 0;
 #pragma unroll
@@ -555,8 +555,12 @@ double prob_impl(qslhip_state *h, int t, unsigned outcome)
     h->red.ensure((kRedBlocks + 8) * sizeof(double));
     double *part = static_cast<double *>(h->red.ptr);
     double *res = part + kRedBlocks;
-    hipLaunchKernelGGL(k_prob_partial<T>, dim3(grid), dim3(kBlock), 0, h->stream,
-                       static_cast<const cplx<T> *>(h->amps), nitems, t, outcome, part);
+    if ((int)h->n >= tuning().stream_min_qubits)
+        hipLaunchKernelGGL((k_prob_partial<T, true>), dim3(grid), dim3(kBlock), 0, h->stream,
+                           static_cast<const cplx<T> *>(h->amps), nitems, t, outcome, part);
+    else
+        hipLaunchKernelGGL((k_prob_partial<T, false>), dim3(grid), dim3(kBlock), 0, h->stream,
+                           static_cast<const cplx<T> *>(h->amps), nitems, t, outcome, part);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, h->stream, part,
                        (int)grid, res);
     QH_CHECK(hipGetLastError());
@@ -618,17 +622,24 @@ double device_norm2(qslhip_state *h)
     return h->prec == 8 ? prob_impl<double>(h, -1, 0) : prob_impl<float>(h, -1, 0);
 }
 
-void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome, double factor)
+template <typename T, bool S>
+void collapse_impl(qslhip_state *h, unsigned t, unsigned outcome, double factor)
 {
     unsigned grid = grid_for(h->namps, kBlock * 4);
-    if (h->prec == 8)
-        hipLaunchKernelGGL(k_collapse<double>, dim3(grid), dim3(kBlock), 0, h->stream,
-                           static_cast<cplx<double> *>(h->amps), h->namps, t, outcome,
-                           factor);
-    else
-        hipLaunchKernelGGL(k_collapse<float>, dim3(grid), dim3(kBlock), 0, h->stream,
-                           static_cast<cplx<float> *>(h->amps), h->namps, t, outcome,
-                           factor);
+    hipLaunchKernelGGL((k_collapse<T, S>), dim3(grid), dim3(kBlock), 0, h->stream,
+                       static_cast<cplx<T> *>(h->amps), h->namps, t, outcome, factor);
+}
+
+void launch_collapse(qslhip_state *h, unsigned t, unsigned outcome, double factor)
+{
+    const bool stream = (int)h->n >= tuning().stream_min_qubits;
+    if (h->prec == 8) {
+        if (stream) collapse_impl<double, true>(h, t, outcome, factor);
+        else collapse_impl<double, false>(h, t, outcome, factor);
+    } else {
+        if (stream) collapse_impl<float, true>(h, t, outcome, factor);
+        else collapse_impl<float, false>(h, t, outcome, factor);
+    }
     QH_CHECK(hipGetLastError());
 }
 
@@ -696,8 +707,12 @@ void sample_impl(qslhip_state *h, const double *u, uint64_t nshots, uint64_t *ou
         hipLaunchKernelGGL(k_chunk_from_subs, dim3(grid_for(nchunks, kBlock)), dim3(kBlock), 0,
                            h->stream, subs, nchunks, pre);
     } else {
-        hipLaunchKernelGGL(k_sub_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0, h->stream,
-                           st, namps, subs, pre);
+        if ((int)h->n >= tuning().stream_min_qubits)
+            hipLaunchKernelGGL((k_sub_sums<T, true>), dim3((unsigned)nchunks), dim3(kBlock), 0,
+                               h->stream, st, namps, subs, pre);
+        else
+            hipLaunchKernelGGL((k_sub_sums<T, false>), dim3((unsigned)nchunks), dim3(kBlock), 0,
+                               h->stream, st, namps, subs, pre);
     }
     scan_exclusive(h, pre, nchunks);
     QH_CHECK(hipGetLastError());
@@ -802,8 +817,8 @@ void sample_noalloc_impl(qslhip_state *h, const double *u, uint64_t nshots, uint
     double *du = subs + kNoAllocChunks * kSubsPerChunk;
     uint64_t *dout = reinterpret_cast<uint64_t *>(du + kNoAllocMaxShots);
     const cplx<T> *st = static_cast<const cplx<T> *>(h->amps);
-    hipLaunchKernelGGL(k_sub_sums<T>, dim3((unsigned)nchunks), dim3(kBlock), 0, h->stream, st,
-                       namps, subs, pre);
+    hipLaunchKernelGGL((k_sub_sums<T, false>), dim3((unsigned)nchunks), dim3(kBlock), 0,
+                       h->stream, st, namps, subs, pre);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, h->stream, pre, nchunks);
     QH_CHECK(hipMemcpyAsync(du, u, nshots * sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (sizeof(T) == 4)

@@ -190,8 +190,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
             }
         }
         if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);
-#pragma unroll
-        for (int k = 0; k < R; k++) st[gbase | tthread | kpart[k]] = v[k];
+        reg_store<T, R>(v, st, gbase | tthread, kpart, ta);
         if (!more) break;
         g = gn;
     }
@@ -362,6 +361,7 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
     ta.gfix = 0;
     ta.zmask = 0;
     ta.zval = 0;
+    ta.stream = n >= (unsigned)tuning().stream_min_qubits;
     int pos = 0;
     for (unsigned q = 0; q < 64; q++) {
         if (!((p.tile_mask >> q) & 1ull)) continue;

@@ -44,6 +44,8 @@ struct RegTileArgs {
                                  // pass over every group
     uint64_t zmask, zval;        // amplitudes whose bits under zmask differ
                                  // from zval are zero and not in memory yet
+    uint32_t stream;             // nonzero: streaming (nontemporal) loads and
+                                 // stores, for states far larger than the caches
 };
 
 // The sampling sums of the amplitudes about to be stored, in the pass's
@@ -86,15 +88,32 @@ __device__ __forceinline__ void reg_load(cplx<T> (&v)[R], const cplx<T> *st, uin
         for (int k = 0; k < R; k++) {
             const uint64_t i = idx0 | kpart[k];
             if ((i & ta.zmask) == ta.zval) {
-                v[k] = st[i];
+                v[k] = ta.stream ? amp_ld<true>(st + i) : amp_ld<false>(st + i);
             } else {
                 v[k].re = T(0);
                 v[k].im = T(0);
             }
         }
+    } else if (ta.stream) {
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = amp_ld<true>(st + (idx0 | kpart[k]));
     } else {
 #pragma unroll
-        for (int k = 0; k < R; k++) v[k] = st[idx0 | kpart[k]];
+        for (int k = 0; k < R; k++) v[k] = amp_ld<false>(st + (idx0 | kpart[k]));
+    }
+}
+
+// The R amplitudes back to state indices idx0 | kpart[k].
+template <typename T, int R>
+__device__ __forceinline__ void reg_store(const cplx<T> (&v)[R], cplx<T> *st, uint64_t idx0,
+                                          const uint64_t (&kpart)[R], const RegTileArgs &ta)
+{
+    if (ta.stream) {
+#pragma unroll
+        for (int k = 0; k < R; k++) amp_st<true>(st + (idx0 | kpart[k]), v[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < R; k++) amp_st<false>(st + (idx0 | kpart[k]), v[k]);
     }
 }
 

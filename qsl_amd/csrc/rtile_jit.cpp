@@ -174,8 +174,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
     }
     o << "        }\n"
          "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);\n"
-         "#pragma unroll\n"
-         "        for (int k = 0; k < R; k++) st[gbase | tthread | kpart[k]] = v[k];\n"
+         "        reg_store<T, R>(v, st, gbase | tthread, kpart, ta);\n"
          "        if (!more) break;\n"
          "        g = gn;\n"
          "    }\n}\n";
