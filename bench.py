@@ -334,10 +334,13 @@ def bench_c5(layers34=30, layers30=30):
                            "single-gate tile ops included in the time)"}
     # the first 6 layers on FP32 register-tile passes, reordered (every pass
     # is compiled at run time, so the comparison stays at 6 layers): the
-    # first application interprets, the second compiles, the third is timed
+    # first two applications interpret (the second queues the structures
+    # for the compiler thread), the third is timed compiled
     ops = ops[: 6 * len(ops) // layers34]
     times = []
-    for _ in range(3):
+    for i in range(3):
+        if i == 2:
+            L.jit_wait()
         q.reset()
         q.sync()
         ev.record(e0, st)
@@ -346,8 +349,9 @@ def bench_c5(layers34=30, layers30=30):
         times.append(ev.elapsed_ms(e0, e1))
     npass = len(L.plan_order(n, 4, ops, L.MODE_REORDER, cap=1 << 16)[0])
     norm_err = abs(q.norm() - 1)
-    for _ in range(2):  # on the dense result: every group; interpret, then compile
+    for _ in range(2):  # on the dense result: every group; interpret, then queue
         q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    L.jit_wait()
     q.sync()
     ev.record(e0, st)
     q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
@@ -464,13 +468,18 @@ def main():
         # the last pass of each application writes its run sums, as in the
         # timed steps
         q.sampleAllOutcomes(np.array([0.5]))
-        # the first application interprets, the second compiles each pass
-        # before launching it, the third runs the compiled passes back to back
+        # the first two applications interpret (the second queues each pass
+        # structure for the compiler thread), the third runs the compiled
+        # passes back to back
         for rep in (1, 2, 3):
+            if rep == 3:
+                L.jit_wait()
             print(f"profile: reordered circuit ({rep})", flush=True)
             q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
             q.sync()
         for rep in (1, 2, 3):
+            if rep == 3:
+                L.jit_wait()
             print(f"profile: program-order circuit ({rep})", flush=True)
             q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
             q.sync()
@@ -507,6 +516,9 @@ def main():
         out = step()
         q.sync()
         warm_ms.append(round(1000 * (time.perf_counter() - tw), 1))
+    # the second warmup queued the repeating pass structures for the
+    # library's compiler thread; the timed steps run them compiled
+    L.jit_wait()
     jit_passes = L.tuning_get("jit_compiled") - jit0
     q.sync()
     t0 = time.perf_counter()
@@ -527,9 +539,12 @@ def main():
     q.sync()
     gate_ms = ev.elapsed_ms(e0, e1)
     # program order: the first application interprets the decoded ops, the
-    # second compiles the passes (untimed), the third runs them compiled
+    # second interprets and queues the passes for compilation, the third runs
+    # them compiled
     fused_runs = []
-    for _ in range(3):
+    for i in range(3):
+        if i == 2:
+            L.jit_wait()
         q.reset()
         q.sync()
         ev.record(e0, s)
@@ -551,6 +566,7 @@ def main():
     # interprets, the second compiles, the third is timed
     for _ in range(2):
         q.apply_ops(ops, mode=hl_mode)
+    L.jit_wait()
     q.sync()
     ev.record(e0, s)
     q.apply_ops(ops, mode=hl_mode)

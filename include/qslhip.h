@@ -178,6 +178,12 @@ int qslhip_plan_mode(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t 
 int qslhip_jit_source(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t nops, int mode,
                       uint64_t pass, char *buf, size_t cap, uint64_t *len);
 int qslhip_jit_check(const char *src, const char *arch, char *log, size_t cap);
+/* Passes compiled at run time (tuning "jit" = 1) are compiled on a host
+ * thread of the library once their structure repeats; until a module is
+ * ready the pass runs interpreted, with the same results.  qslhip_jit_wait
+ * blocks until every queued structure is compiled (no reference
+ * counterpart; for callers that time steady state). */
+int qslhip_jit_wait(void);
 
 /* ---- tuning (no reference counterpart) ------------------------------------
  * Keys: "logt_f64", "low_f64", "logt_f32", "low_f32" (LDS tile width and the

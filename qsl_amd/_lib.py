@@ -68,6 +68,7 @@ SIGNATURES = {
     "qslhip_jit_source": (_I, [_U, _I, _vp, _u64, _I, _u64, C.c_char_p, C.c_size_t,
                                C.POINTER(_u64)]),
     "qslhip_jit_check": (_I, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
+    "qslhip_jit_wait": (_I, []),
     "qslhip_reset": (_I, [_vp]),
     "qslhip_set_state": (_I, [_vp, _vp, _u64]),
     "qslhip_get_state": (_I, [_vp, _vp, _u64]),
@@ -207,6 +208,11 @@ def tuning_get(key):
     v = _I(0)
     check(lib().qslhip_tuning_get(key.encode(), C.byref(v)))
     return v.value
+
+
+def jit_wait():
+    """Block until every pass structure queued for compilation is compiled."""
+    check(lib().qslhip_jit_wait())
 
 
 def tuning_set(key, value):

@@ -1089,6 +1089,11 @@ int qslhip_reallocate(qslhip_t h)
     });
 }
 
+int qslhip_jit_wait(void)
+{
+    return guarded([&] { jit_wait_idle(); });
+}
+
 int qslhip_sample_all_noalloc(qslhip_t h, const double *u, uint64_t nshots, uint64_t *out)
 {
     return guarded([&] {

@@ -114,6 +114,9 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "jit_waves") *value = t.jit_waves;
         else if (k == "stream_min_qubits") *value = t.stream_min_qubits;
         else if (k == "lane_gates") *value = t.lane_gates;
+        else if (k == "np_adj_max") *value = t.np_adj_max;
+        else if (k == "np_adj_policy") *value = t.np_adj_policy;
+        else if (k == "np_stream_min_q") *value = t.np_stream_min_q;
         else if (k == "np_bonds") *value = t.np_bonds;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
@@ -164,6 +167,16 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "np_bonds") {
             QH_REQUIRE(value >= 1 && value <= 4, QSLHIP_EINVAL, "np_bonds must be in [1, 4]");
             t.np_bonds = value;
+        } else if (k == "np_stream_min_q") {
+            QH_REQUIRE(value >= 0 && value <= 64, QSLHIP_EINVAL,
+                       "np_stream_min_q must be in [0, 64]");
+            t.np_stream_min_q = value;
+        } else if (k == "np_adj_policy") {
+            QH_REQUIRE(value >= 0 && value <= 3, QSLHIP_EINVAL, "np_adj_policy must be in [0, 3]");
+            t.np_adj_policy = value;
+        } else if (k == "np_adj_max") {
+            QH_REQUIRE(value >= -1 && value <= 62, QSLHIP_EINVAL, "np_adj_max must be in [-1, 62]");
+            t.np_adj_max = value;
         } else if (k == "lane_gates") {
             QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "lane_gates must be 0 or 1");
             t.lane_gates = value;

@@ -233,6 +233,15 @@ struct Tuning {
                                  // 1 once a pass structure repeats, 2 always
     int np_bonds = 1;            // NP: swap-like gates on disjoint pairs per pass (1: one
                                  // each; grouping measured slower for config 3)
+    int np_adj_max = 4;          // NP: adjacent swap-like gates on qubits q, q+1 with
+                                 // q <= this (and q+1 in the low table part) run one
+                                 // element per lane with cross-lane exchange (-1: never)
+    int np_adj_policy = 2;       // that kernel's memory policy: 0 pair members only,
+                                 // 1 members with streaming accesses, 2 whole chunks,
+                                 // 3 whole chunks streaming
+    int np_stream_min_q = 64;    // NP gates whose lowest qubit is at least this use
+                                 // streaming accesses (states of stream_min_qubits+);
+                                 // off: on C(32,16) FP64 they run 5-10% slower
     int lane_gates = 1;          // one-qubit gates on targets 0..5 by cross-lane
                                  // exchange (0: one-op LDS tiles / streaming pairs)
     int stream_min_qubits = 26;  // states of at least this many qubits (1 GiB FP64)
@@ -243,6 +252,7 @@ struct Tuning {
 };
 Tuning &tuning();
 extern int g_jit_compiled, g_jit_failed;  // passes compiled at run time / failed
+void jit_wait_idle();  // blocks until the compiler thread has no queued pass
 extern int g_run_sums_used;  // samplings that took a gate pass's run sums
 
 inline unsigned grid_for(uint64_t items, unsigned per_block)

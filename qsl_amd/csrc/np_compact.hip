@@ -37,6 +37,7 @@ struct NPTables {
     uint32_t *lo_list = nullptr;      // [2^L] strings grouped by weight, ascending
     uint32_t *rank_lo = nullptr;      // [2^L] index of a string inside its weight
     uint32_t *class_start = nullptr;  // [L + 2] offset of each weight in lo_list
+    uint64_t *binom = nullptr;        // [64 * 64] C(q, m) at q * 64 + m
     NPBlk *blk = nullptr;             // block map
     uint32_t nblk = 0;
     std::vector<uint64_t> h_seg_off;  // host copies
@@ -67,6 +68,7 @@ struct NPDev {
     const uint32_t *rank_lo;
     const uint32_t *class_start;
     const NPBlk *blk;
+    const uint64_t *binom;
     unsigned L, k;
     uint64_t lomask;
 };
@@ -80,6 +82,7 @@ NPDev dev_view(const qslhip_state *h)
     d.rank_lo = t->rank_lo;
     d.class_start = t->class_start;
     d.blk = t->blk;
+    d.binom = t->binom;
     d.L = t->L;
     d.k = h->k;
     d.lomask = (1ull << t->L) - 1;
@@ -89,6 +92,20 @@ NPDev dev_view(const qslhip_state *h)
 __device__ __forceinline__ uint64_t np_rank(const NPDev &d, uint64_t x)
 {
     return d.seg_off[x >> d.L] + d.rank_lo[x & d.lomask];
+}
+
+// The rank of x with the one of the adjacent pair (q0, q1) moved from q0 to
+// q1 (x has bit q0 set and q1 clear), from the rank r of x.  With q the
+// lower of the two and j - 1 ones of x below q, the moved one is the j-th
+// one; its colex term C(q, j) becomes C(q + 1, j), a change of C(q, j - 1)
+// (Pascal's rule): up by it when q1 = q0 + 1, down when q1 = q0 - 1.  No
+// rank table is read.
+__device__ __forceinline__ uint64_t adj_partner(const NPDev &d, uint64_t x, uint64_t r,
+                                                unsigned q0, unsigned q1)
+{
+    const unsigned q = q0 < q1 ? q0 : q1;
+    const uint64_t m = d.binom[q * 64 + __popcll(x & ((1ull << q) - 1))];
+    return q1 > q0 ? r + m : r - m;
 }
 
 // Per-block segment context.
@@ -119,7 +136,7 @@ __device__ __forceinline__ Seg seg_of(const NPDev &d)
 //   PC_SWAPLIKE / PC_FSWAP  owner x (bit q0 = 1, bit q1 = 0) updates itself
 //            and its partner x ^ bit q0 ^ bit q1 (np/gates.cpp:236-415);
 //            fswap also negates the |11> strings (swap then controlZ).
-template <typename T, int CLASS>
+template <typename T, int CLASS, bool S>
 __global__ __launch_bounds__(kBlock) void k_np_gate(cplx<T> *__restrict__ amp, NPDev d,
                                                     int body, unsigned q0, unsigned q1,
                                                     T c, T s)
@@ -127,6 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_np_gate(cplx<T> *__restrict__ amp, N
     const Seg g = seg_of(d);
     const uint64_t b0 = 1ull << q0, b1 = 1ull << q1;
     const uint64_t hix = (uint64_t)g.hi << d.L;
+    const bool adjacent = q0 + 1 == q1 || q1 + 1 == q0;
     // Whole-segment decisions when the qubits are in the high part.
     if (CLASS == PC_ONE && q0 >= d.L && !(hix & b0)) return;
     if (CLASS == PC_DIAG2 && ((q0 >= d.L && !(hix & b0)) || (q1 >= d.L && !(hix & b1))))
@@ -142,39 +160,118 @@ __global__ __launch_bounds__(kBlock) void k_np_gate(cplx<T> *__restrict__ amp, N
         uint64_t r = g.off + i;
         if constexpr (CLASS == PC_ONE) {
             if (x & b0) {
-                cplx<T> a = amp[r];
+                cplx<T> a = amp_ld<S>(amp + r);
                 one_update<T>(body, a, c, s);
-                amp[r] = a;
+                amp_st<S>(amp + r, a);
             }
         } else if constexpr (CLASS == PC_PAIR1) {
-            cplx<T> a = amp[r];
+            cplx<T> a = amp_ld<S>(amp + r);
             rz_half<T>((x & b0) != 0, a, c, s);
-            amp[r] = a;
+            amp_st<S>(amp + r, a);
         } else if constexpr (CLASS == PC_DIAG2) {
             if ((x & b0) && (x & b1)) {
-                cplx<T> a = amp[r];
+                cplx<T> a = amp_ld<S>(amp + r);
                 one_update<T>(body, a, c, s);
-                amp[r] = a;
+                amp_st<S>(amp + r, a);
             }
         } else if constexpr (CLASS == PC_CTRL) {
             if (x & b0) {
-                cplx<T> a = amp[r];
+                cplx<T> a = amp_ld<S>(amp + r);
                 rz_half<T>((x & b1) != 0, a, c, s);
-                amp[r] = a;
+                amp_st<S>(amp + r, a);
             }
         } else {  // PC_SWAPLIKE, PC_FSWAP
             if ((x & b0) && !(x & b1)) {
-                uint64_t rp = np_rank(d, x ^ b0 ^ b1);
-                cplx<T> a = amp[r], b = amp[rp];
+                uint64_t rp = adjacent ? adj_partner(d, x, r, q0, q1) : np_rank(d, x ^ b0 ^ b1);
+                cplx<T> a = amp_ld<S>(amp + r), b = amp_ld<S>(amp + rp);
                 pair_update<T>(body, a, b, c, s);
-                amp[r] = a;
-                amp[rp] = b;
+                amp_st<S>(amp + r, a);
+                amp_st<S>(amp + rp, b);
             } else if (CLASS == PC_FSWAP && (x & b0) && (x & b1)) {
-                cplx<T> a = amp[r];
+                cplx<T> a = amp_ld<S>(amp + r);
                 one_update<T>(QSLHIP_PAULI_Z, a, c, s);
-                amp[r] = a;
+                amp_st<S>(amp + r, a);
             }
         }
+    }
+}
+
+// A swap-like gate on an adjacent pair inside the low part of the strings
+// (q_lo + 1 < L), one element per lane.  Within a segment the strings are in
+// numeric order, so the two members of a pair (bits (q+1, q) = 01 and 10,
+// the same other bits) sit C(q, ones below q) ranks apart; for low q that is
+// a few ranks, inside one wavefront.  Every lane loads its own amplitude
+// (contiguous loads across the wave), takes its pair member's amplitude from
+// that member's lane by a cross-lane exchange (or from memory when the
+// member is in another wavefront), applies the pair update of the gate
+// (k_np_gate's arithmetic for both members) and stores its own result; a
+// pair split over two wavefronts is done by its owner alone.  Strings with
+// bits 00 or 11 at (q+1, q) are neither read nor written.
+template <typename T, bool S, bool ALL>
+__global__ __launch_bounds__(kBlock) void k_np_adj(cplx<T> *__restrict__ amp, NPDev d, int body,
+                                                   unsigned q0, unsigned q1, T c, T s)
+{
+    __shared__ uint64_t crow[64];  // C(q, m)
+    const Seg g = seg_of(d);
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+    const unsigned q = q0 < q1 ? q0 : q1;
+    if (threadIdx.x < 64) crow[threadIdx.x] = d.binom[q * 64 + threadIdx.x];
+    __syncthreads();
+    const uint64_t bq = 1ull << q, bq1 = bq << 1, b0 = 1ull << q0;
+    const uint64_t lowm = bq - 1;
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int e = 0; e < kItems; e++) {
+        const uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        const bool in = i < g.size && i < g.i0 + kChunk;
+        uint64_t x = 0;
+        cplx<T> own;
+        own.re = own.im = (T)0;
+        // ALL: every element of the chunk is loaded and stored (whole lines,
+        // no wait for the string before the amplitude load)
+        if (ALL && in) own = amp_ld<S>(amp + g.off + i);
+        if (in) x = hix | d.lo_list[g.cstart + i];
+        // member of a pair: exactly one of bits q, q + 1 set
+        const bool mem = in && (((x & bq) != 0) != ((x & bq1) != 0));
+        if (!ALL && mem) own = amp_ld<S>(amp + g.off + i);
+        // offset to the other member: + m from the 01 string, - m from 10
+        const uint64_t m = mem ? crow[__popcll(x & lowm)] : 0;
+        const bool up = (x & bq) != 0;  // holds the 01 string
+        const int64_t delta = up ? (int64_t)m : -(int64_t)m;
+        const int64_t src = (int64_t)lane + delta;
+        // the other member is in this wavefront's lanes (same item, same
+        // block) when its segment index lies in the chunk and its lane in
+        // [0, 64)
+        const int64_t io = (int64_t)i + delta;
+        const bool local = mem && src >= 0 && src < 64 && io >= (int64_t)g.i0 &&
+                           io < (int64_t)(g.i0 + kChunk) && io < (int64_t)g.size;
+        const int srcl = local ? (int)src : (int)lane;
+        cplx<T> other;
+        other.re = __shfl(own.re, srcl);
+        other.im = __shfl(own.im, srcl);
+        // owner: bit q0 set (and q1 clear); the pair update takes it first
+        const bool owner = mem && (x & b0) != 0;
+        // A pair split over two wavefronts is the owner's alone: it reads
+        // and writes both members, and the other member's lane leaves it
+        // (its lane could otherwise read the owner after the owner's store).
+        if (mem && !local && owner) other = amp[g.off + (uint64_t)io];
+        if (mem && (local || owner)) {
+            cplx<T> a, b;
+            a.re = owner ? own.re : other.re;
+            a.im = owner ? own.im : other.im;
+            b.re = owner ? other.re : own.re;
+            b.im = owner ? other.im : own.im;
+            pair_update<T>(body, a, b, c, s);
+            cplx<T> out;
+            out.re = owner ? a.re : b.re;
+            out.im = owner ? a.im : b.im;
+            own = out;
+            if (!ALL) amp_st<S>(amp + g.off + i, out);
+            if (!local) amp[g.off + (uint64_t)io] = b;
+        }
+        // the untouched strings are written back unchanged; a split pair's
+        // non-owner member is the owner's to write
+        if (ALL && in && !(mem && !local && !owner)) amp_st<S>(amp + g.off + i, own);
     }
 }
 
@@ -468,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_np_search(const double *__restrict__
     out[i] = (a << d.L) | d.lo_list[d.class_start[w] + idx];
 }
 
-template <typename T>
+template <typename T, bool S>
 void gate_impl(qslhip_state *h, const GateDesc &g)
 {
     NPDev d = dev_view(h);
@@ -478,28 +575,44 @@ void gate_impl(qslhip_state *h, const GateDesc &g)
     const int body = body_of(g.kind);
     switch (g.kind) {
     case QSLHIP_PAULI_Z: case QSLHIP_PHASE:
-        hipLaunchKernelGGL((k_np_gate<T, PC_ONE>), grid, blk, 0, h->stream, amp, d, body,
+        hipLaunchKernelGGL((k_np_gate<T, PC_ONE, S>), grid, blk, 0, h->stream, amp, d, body,
                            g.q0, 0u, c, s);
         break;
     case QSLHIP_ROT_Z:
-        hipLaunchKernelGGL((k_np_gate<T, PC_PAIR1>), grid, blk, 0, h->stream, amp, d, body,
+        hipLaunchKernelGGL((k_np_gate<T, PC_PAIR1, S>), grid, blk, 0, h->stream, amp, d, body,
                            g.q0, 0u, c, s);
         break;
     case QSLHIP_CZ: case QSLHIP_CPHASE:
-        hipLaunchKernelGGL((k_np_gate<T, PC_DIAG2>), grid, blk, 0, h->stream, amp, d, body,
+        hipLaunchKernelGGL((k_np_gate<T, PC_DIAG2, S>), grid, blk, 0, h->stream, amp, d, body,
                            g.q0, g.q1, c, s);
         break;
     case QSLHIP_CROT_Z:
-        hipLaunchKernelGGL((k_np_gate<T, PC_CTRL>), grid, blk, 0, h->stream, amp, d, body,
+        hipLaunchKernelGGL((k_np_gate<T, PC_CTRL, S>), grid, blk, 0, h->stream, amp, d, body,
                            g.q0, g.q1, c, s);
         break;
     case QSLHIP_SWAP: case QSLHIP_NP_ROT_X: case QSLHIP_NP_ROT_Y:
-    case QSLHIP_NP_HADAMARD:
-        hipLaunchKernelGGL((k_np_gate<T, PC_SWAPLIKE>), grid, blk, 0, h->stream, amp, d,
+    case QSLHIP_NP_HADAMARD: {
+        // adjacent pairs low in the strings: members a few ranks apart, one
+        // element per lane with cross-lane exchange
+        const unsigned ql = g.q0 < g.q1 ? g.q0 : g.q1, qh = g.q0 < g.q1 ? g.q1 : g.q0;
+        if (qh == ql + 1 && qh < h->npt->L && (int)ql <= tuning().np_adj_max) {
+            const int pol = tuning().np_adj_policy;
+#define ADJ(SV, AV)                                                                     \
+    hipLaunchKernelGGL((k_np_adj<T, SV, AV>), grid, blk, 0, h->stream, amp, d, body, g.q0, \
+                       g.q1, c, s)
+            if (pol == 0) ADJ(false, false);
+            else if (pol == 1) ADJ(true, false);
+            else if (pol == 2) ADJ(false, true);
+            else ADJ(true, true);
+#undef ADJ
+            break;
+        }
+        hipLaunchKernelGGL((k_np_gate<T, PC_SWAPLIKE, S>), grid, blk, 0, h->stream, amp, d,
                            body, g.q0, g.q1, c, s);
         break;
+    }
     case QSLHIP_FSWAP:
-        hipLaunchKernelGGL((k_np_gate<T, PC_FSWAP>), grid, blk, 0, h->stream, amp, d, body,
+        hipLaunchKernelGGL((k_np_gate<T, PC_FSWAP, S>), grid, blk, 0, h->stream, amp, d, body,
                            g.q0, g.q1, c, s);
         break;
     default:
@@ -589,6 +702,12 @@ void np_init(qslhip_state *h)
     QH_CHECK(hipMalloc(&t->rank_lo, nlo * sizeof(uint32_t)));
     QH_CHECK(hipMalloc(&t->class_start, (L + 2) * sizeof(uint32_t)));
     QH_CHECK(hipMalloc(&t->blk, blks.size() * sizeof(NPBlk)));
+    std::vector<uint64_t> binom(64 * 64);
+    for (unsigned q = 0; q < 64; q++)
+        for (unsigned m = 0; m < 64; m++) binom[q * 64 + m] = choose_host(q, m);
+    QH_CHECK(hipMalloc(&t->binom, binom.size() * sizeof(uint64_t)));
+    QH_CHECK(hipMemcpy(t->binom, binom.data(), binom.size() * sizeof(uint64_t),
+                       hipMemcpyHostToDevice));
     QH_CHECK(hipMemcpy(t->seg_off, t->h_seg_off.data(), (nseg + 1) * sizeof(uint64_t),
                        hipMemcpyHostToDevice));
     QH_CHECK(hipMemcpy(t->lo_list, t->h_lo_list.data(), nlo * sizeof(uint32_t),
@@ -611,14 +730,27 @@ void np_free(qslhip_state *h)
     if (t->rank_lo) (void)hipFree(t->rank_lo);
     if (t->class_start) (void)hipFree(t->class_start);
     if (t->blk) (void)hipFree(t->blk);
+    if (t->binom) (void)hipFree(t->binom);
     delete t;
     h->npt = nullptr;
 }
 
 void np_launch_gate(qslhip_state *h, const GateDesc &g)
 {
-    if (h->prec == 8) gate_impl<double>(h, g);
-    else gate_impl<float>(h, g);
+    // Streaming accesses where each lane's run is long: a gate whose lowest
+    // qubit is q touches runs of C(q, .) consecutive ranks, whole segments
+    // when both qubits are in the high part of the strings.
+    unsigned ql = g.q0;
+    if (is_gate2(g.kind) && g.q1 < ql) ql = g.q1;
+    const bool stream = (int)h->n >= tuning().stream_min_qubits &&
+                        (int)ql >= tuning().np_stream_min_q;
+    if (h->prec == 8) {
+        if (stream) gate_impl<double, true>(h, g);
+        else gate_impl<double, false>(h, g);
+    } else {
+        if (stream) gate_impl<float, true>(h, g);
+        else gate_impl<float, false>(h, g);
+    }
 }
 
 // The class of an elementwise (diagonal) NP gate, or -1.

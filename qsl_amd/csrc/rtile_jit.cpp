@@ -16,11 +16,15 @@
 // results are the same bit for bit.
 #include <hip/hiprtc.h>
 
+#include <condition_variable>
 #include <cstdio>
+#include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 #include "rtile.h"
@@ -29,6 +33,10 @@ namespace qslhip {
 namespace {
 
 #include "jit_headers.inc"  // kJitHeaders: {name, text} of the device headers
+
+}  // namespace
+extern int g_jit_compiled, g_jit_failed;
+namespace {
 
 struct Compiled {
     hipModule_t mod = nullptr;
@@ -40,6 +48,11 @@ std::mutex g_mu;
 std::unordered_map<std::string, int> g_seen;               // pass source -> applications
 std::map<std::pair<int, std::string>, Compiled> g_cache;   // (device, source) -> module
 constexpr size_t kMaxCached = 1024;
+// Structures seen once are remembered so that a repeat can be recognised;
+// a workload of ever-new structures would grow this without end, so it is
+// cleared when it reaches this size (a structure then needs two more
+// applications before it is compiled).
+constexpr size_t kMaxSeen = 4096;
 
 std::string hex(uint64_t x)
 {
@@ -230,6 +243,79 @@ bool compile(int dev, const std::string &src, Compiled &out)
     return true;
 }
 
+// Compiles pass structures on a host thread of its own (jit mode 1): the
+// application that finds a structure repeating queues it and runs the pass
+// interpreted; a later application finds the module in the cache.  The
+// thread is joined when the library unloads (a compile in progress
+// finishes first).
+class Compiler {
+public:
+    ~Compiler()
+    {
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+
+    // Blocks until every queued structure is compiled (or failed).
+    void wait_idle()
+    {
+        std::unique_lock<std::mutex> lk(g_mu);
+        idle_.wait(lk, [this] { return pending_.empty(); });
+    }
+
+    // Called with g_mu held.
+    void enqueue(int dev, const std::string &src)
+    {
+        auto key = std::make_pair(dev, src);
+        if (pending_.count(key)) return;
+        pending_.insert(key);
+        jobs_.push_back(key);
+        if (!th_.joinable()) th_ = std::thread([this] { run(); });
+        cv_.notify_one();
+    }
+
+private:
+    void run()
+    {
+        std::unique_lock<std::mutex> lk(g_mu);
+        for (;;) {
+            cv_.wait(lk, [this] { return stop_ || !jobs_.empty(); });
+            if (stop_) return;
+            const auto key = jobs_.front();
+            jobs_.pop_front();
+            lk.unlock();
+            Compiled c;
+            bool ok = hipSetDevice(key.first) == hipSuccess && compile(key.first, key.second, c);
+            lk.lock();
+            if (!ok) {
+                c.failed = true;
+                g_jit_failed++;
+            } else {
+                g_jit_compiled++;
+            }
+            if (g_cache.size() < kMaxCached) g_cache.emplace(key, c);
+            pending_.erase(key);
+            if (pending_.empty()) idle_.notify_all();
+        }
+    }
+
+    std::thread th_;
+    std::condition_variable cv_, idle_;
+    std::deque<std::pair<int, std::string>> jobs_;
+    std::set<std::pair<int, std::string>> pending_;
+    bool stop_ = false;
+};
+
+Compiler &compiler()
+{
+    static Compiler c;
+    return c;
+}
+
 }  // namespace
 
 std::string jit_pass_source(unsigned n, int prec, const TilePass &p, int threads, bool exact,
@@ -247,6 +333,8 @@ bool jit_check_source(const std::string &src, const std::string &arch, std::stri
 }
 
 int g_jit_compiled = 0, g_jit_failed = 0;  // counters (tuning keys jit_compiled / jit_failed)
+
+void jit_wait_idle() { compiler().wait_idle(); }
 
 void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef)
 {
@@ -271,13 +359,18 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     hipFunction_t fn = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        const int seen = ++g_seen[src];
-        if (mode == 1 && seen < 2) return false;  // compile a structure once it repeats
         auto key = std::make_pair(h->device, src);
         auto it = g_cache.find(key);
         if (it == g_cache.end()) {
             if (g_cache.size() >= kMaxCached) return false;
-            Compiled c;
+            if (mode == 1) {
+                // compile a structure once it repeats, on the compiler
+                // thread; this application runs interpreted
+                if (g_seen.size() >= kMaxSeen) g_seen.clear();
+                if (++g_seen[src] >= 2) compiler().enqueue(h->device, src);
+                return false;
+            }
+            Compiled c;  // mode 2: compile now
             if (!compile(h->device, src, c)) {
                 c.failed = true;
                 g_jit_failed++;

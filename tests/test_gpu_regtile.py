@@ -231,8 +231,10 @@ def test_compiled_passes_match_oracle(tuning, jit_mode, dtype, cfg):
 @pytest.mark.parametrize("mode", [qsl_amd.MODE_FUSED, qsl_amd.MODE_REORDER])
 def test_compiled_pass_equals_interpreted_pass(tuning, jit_mode, mode):
     # jit = 1: the first application interprets the decoded ops, the second
-    # runs the compiled pass; the same arithmetic in the same order gives the
-    # same bits, also with new angles (the compiled pass is angle-free)
+    # queues the structure for the compiler thread (and still interprets),
+    # the third runs the compiled pass; the same arithmetic in the same order
+    # gives the same bits, also with new angles (the compiled pass is
+    # angle-free)
     tuning(256, 8, 11, 3)
     jit_mode(1)
     n = 19
@@ -247,7 +249,11 @@ def test_compiled_pass_equals_interpreted_pass(tuning, jit_mode, mode):
     q.setState(st0)
     q.apply_ops(ops, mode=mode)
     assert np.array_equal(q.getState().view(np.uint8), first.view(np.uint8))
+    L.jit_wait()
     assert L.tuning_get("jit_compiled") > before
+    q.setState(st0)
+    q.apply_ops(ops, mode=mode)  # compiled passes
+    assert np.array_equal(q.getState().view(np.uint8), first.view(np.uint8))
     ops2 = ops.copy()
     ops2["angle"] = rng.uniform(-6, 6, len(ops))
     q.setState(st0)
