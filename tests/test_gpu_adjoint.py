@@ -18,6 +18,7 @@ import pytest
 
 import qsl_amd
 from qsl_amd import workloads as W
+from qsl_amd import _lib as L
 from oracle.pyoracle import Oracle, OP_DTYPE, GATE1, GATE2
 
 pytestmark = pytest.mark.gpu
@@ -152,7 +153,7 @@ def test_batch_vs_fp64_plain(sizes):
 
 @pytest.mark.parametrize("sizes", [[1, 2, 3, 5], [8, 12], [13, 15]])
 def test_batch_every_gate_kind_vs_fp64_plain(sizes):
-    # the batch kernel applies each pass's ops in register blocks of two
+    # the batch kernel applies each pass's ops in register blocks of three
     # tile positions: every one- and two-qubit kind (controls, swaps, the
     # number-preserving rotations, diagonal gates on qubits outside the
     # tile), against the FP64 per-circuit path, and the energy against the
@@ -175,6 +176,34 @@ def test_batch_every_gate_kind_vs_fp64_plain(sizes):
         assert abs(e64 - oracle_energy(n, ops, terms)) < 1e-12, n
         gc = g[off:off + len(ops)]
         assert np.abs(gc - g64).max() < 1e-4, (n, np.abs(gc - g64).max())
+        off += len(ops)
+
+
+def test_batch_each_kind_at_each_placement():
+    # every kind on every qubit (pair) of a 3-qubit circuit, between
+    # rotations on all three qubits: the whole state is one 8-amplitude
+    # register block, so each kind runs at each of its block-position
+    # patterns, forward and inverse
+    rng = np.random.default_rng(7)
+    n = 3
+    terms = np.array([(0.7, 0b011, 0b101), (0.4, 0b100, 0b010), (0.3, 0b001, 0b110)], PAULI)
+    circuits = []
+    for k in GATE1 + GATE2:
+        places = ([(q, 0) for q in range(n)] if k < 16 else
+                  [(a, b) for a in range(n) for b in range(n) if a != b])
+        for a, b in places:
+            seq = [(L.ROT_X, 0, 0), (L.ROT_Y, 1, 0), (L.ROT_X, 2, 0), (k, a, b),
+                   (L.ROT_Y, 0, 0), (L.ROT_X, 1, 0), (L.ROT_Y, 2, 0)]
+            ops = np.zeros(len(seq), OP_DTYPE)
+            for i, (kk, q0, q1) in enumerate(seq):
+                ops[i] = (kk, q0, q1, 0, float(rng.uniform(-3, 3)))
+            circuits.append((n, ops, terms))
+    e, g = qsl_amd.Batch(circuits).run()
+    off = 0
+    for c, (nn, ops, t) in enumerate(circuits):
+        e64, g64 = qsl_amd.Qubits(nn).expvalGrad(ops, t)
+        assert abs(e[c] - e64) < 1e-5, (c, ops[3])
+        assert np.abs(g[off:off + len(ops)] - g64).max() < 1e-4, (c, ops[3])
         off += len(ops)
 
 
