@@ -78,7 +78,18 @@ def host_cores():
     return max(1, min(16, n))
 
 
-def cpu_baseline(ops, gpu_state_fn, budget_s=20.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(ops, gpu_state_fn, budget_s=20.0, leg_budget_s=8.0):
     """The reference's Omp backend (oracle/_ref, compiled from qsl's own
     sources) -- or the oracle's restatement of it when that library is
     absent -- applying the first gates of the same circuit to a 30-qubit
@@ -127,11 +138,73 @@ def cpu_baseline(ops, gpu_state_fn, budget_s=20.0):
     err = float(np.max(np.abs(cpu - gpu)))
     bit_exact = bool(np.array_equal(cpu.view(np.uint8), gpu.view(np.uint8)))
     del cpu, gpu
+    legs = []
+    if kind == "reference":
+        # the reference's own defaults beside it: Qubits<Omp> at its default
+        # 4 threads, and the single-threaded Qubits<Default>
+        for typ, threads, name in ((P.RefLib.OMP, 4, "Qubits<Omp,double>"),
+                                   (P.RefLib.DEFAULT, 1, "Qubits<Default,double>")):
+            sim = None
+            sim = P.RefSim(ref, typ, np.complex128, N, threads if typ == P.RefLib.OMP else 0)
+
+            def apply_leg(op, sim=sim):
+                k = int(op["kind"])
+                if k < 16:
+                    sim.gate1(k, int(op["q0"]), float(op["angle"]))
+                else:
+                    sim.gate2(k, int(op["q0"]), int(op["q1"]), float(op["angle"]))
+            n_leg, t_leg = 0, 0.0
+            for op in ops:
+                t0 = time.perf_counter()
+                apply_leg(op)
+                t_leg += time.perf_counter() - t0
+                n_leg += 1
+                if t_leg > leg_budget_s:
+                    break
+            del sim
+            legs.append({"backend": name, "threads": threads,
+                         "gates_per_s": round(n_leg / t_leg, 3), "gates": n_leg,
+                         "seconds": round(t_leg, 1)})
     return {"value": done / spent, "unit": "gates/s", "cores": cores, "kind": kind,
+            "cpu_model": cpu_model(), "legs": legs,
             "sample": f"first {done} gates of the same 200-gate circuit on a "
                       f"30-qubit FP64 state ({spent:.1f} s of gate time); "
                       f"{'qsl Qubits<Omp,double> built from the reference sources' if kind == 'reference' else 'oracle restatement of the Omp loops'}",
             "max_abs_err_vs_gpu": err, "bit_exact_vs_gpu": bit_exact}
+
+
+def step_pass_bytes(masks, n, amp_bytes=16):
+    """HBM bytes the reordered tile passes move when the circuit starts
+    from a basis state (the library's rule, regtile.hip pass_extras): pass p
+    runs only the groups that agree with the basis outside the qubits
+    earlier tiles held; with every qubit in some tile, the first pass only
+    writes the basis index's group and a later pass reads only what an
+    earlier one wrote; otherwise the first pass writes every group.  The
+    last pass also writes 2^n bytes of run sums when it covers every group."""
+    full = (1 << n) - 1
+    allq = 0
+    for m in masks:
+        allq |= int(m)
+    fill = (allq & full) != full
+    support, total = 0, 0
+    for i, m in enumerate(masks):
+        m = int(m)
+        t = bin(m).count("1")
+        outside = full & ~m
+        if i == 0 and fill:
+            groups, reads = 1 << (n - t), 0  # every group written, none read
+        elif i == 0:
+            groups, reads = 1, 0  # the basis index's group, generated in registers
+        else:
+            groups = 1 << bin(outside & support).count("1")
+            reads = (1 << bin(m & support).count("1")) if not fill else (1 << t)
+        total += groups * ((1 << t) + reads) * amp_bytes
+        support |= m
+    if masks is not None and len(masks):
+        last = int(masks[-1])
+        if (full & ~last & ~support) == 0:
+            total += 1 << n
+    return total
 
 
 def read_traffic():
@@ -369,21 +442,31 @@ def bench_c5(layers34=30, layers30=30):
     del q
     n = 30
     ops, _ = W.c5_haar_layers(n, layers30)
+    # expectation value of H = sum_i Z_i Z_{i+1} + 0.5 sum_i X_i (FP64
+    # accumulation on the device for every precision)
+    terms = np.array([(1.0, 0, 3 << i) for i in range(n - 1)] +
+                     [(0.5, 1 << i, 0) for i in range(n)], L.PAULI_DTYPE)
     q32 = qsl_amd.Qubits(n, dtype=np.complex64)
     q32.apply_ops(ops, mode=qsl_amd.MODE_DENSE)
+    e32 = q32.expval(terms)
     s32 = q32.getState()
     q32.reset()
     q32.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+    e32r = q32.expval(terms)
     s32r = q32.getState()
     del q32
     q64 = qsl_amd.Qubits(n)
     q64.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+    e64 = q64.expval(terms)
     s64 = q64.getState()
     del q64
     res["max_amp_err_30q_vs_fp64"] = float(np.abs(s32.astype(np.complex128) - s64).max())
     res["register_tile_fp32"]["max_amp_err_30q_vs_fp64"] = float(
         np.abs(s32r.astype(np.complex128) - s64).max())
     res["max_amp_30q"] = float(np.abs(s64).max())
+    res["expval_30q"] = {
+        "observable": "sum_i Z_i Z_(i+1) + 0.5 sum_i X_i", "fp64": e64,
+        "dense_mfma_fp32_err": abs(e32 - e64), "register_tile_fp32_err": abs(e32r - e64)}
     return res
 
 
@@ -495,7 +578,8 @@ def main():
         print("profile: done", flush=True)
         return
     shots_u = stream.take(NSHOTS)
-    npasses = len(L.plan_order(N, 8, ops, L.MODE_REORDER)[0])
+    reorder_masks = L.plan_order(N, 8, ops, L.MODE_REORDER)[0]
+    npasses = len(reorder_masks)
     npasses_fused = len(L.plan_order(N, 8, ops, L.MODE_FUSED)[0])
     hl_mode = mode if mode != qsl_amd.MODE_PER_GATE else qsl_amd.MODE_REORDER
     hl_passes = npasses if hl_mode == qsl_amd.MODE_REORDER else npasses_fused
@@ -622,6 +706,8 @@ def main():
     del ref_state, reorder_state
 
     achieved = steady_bytes / (steady_ms * 1e-3) / 1e9  # per launch: both / passes
+    step_bytes = step_pass_bytes(reorder_masks if hl_mode == qsl_amd.MODE_REORDER else
+                                 L.plan_order(N, 8, ops, L.MODE_FUSED)[0], N)
     traffic = read_traffic()
     result = {
         "metric": METRIC,
@@ -691,6 +777,17 @@ def main():
             "fused_hbm_GBps_actual": round(npasses_fused * pass_bytes / (fused_ms * 1e-3) / 1e9, 1),
             "fused_bit_identical_to_per_gate": fused_bit_identical,
             "norm_err": norm_err,
+        },
+        # the same 200 gates on a dense state (every pass reads and writes
+        # all of it), and the HBM rate of the timed step's gate phase counted
+        # from the bytes its passes actually move from |0..0>
+        "dense_gates_per_s": round(NGATES / (steady_ms * 1e-3), 1),
+        "step_gate_phase_hbm": {
+            "bytes": step_bytes, "ms": round(gate_ms, 3),
+            "GBps": round(step_bytes / (gate_ms * 1e-3) / 1e9, 1),
+            "frac": round(step_bytes / (gate_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "rule": "reordered passes from a basis state run only the groups that "
+                    "can be nonzero and read only what an earlier pass wrote",
         },
         "rotateX_f64_GBps_by_target": per_target,
         "sampling": {"shots": NSHOTS, "ms": round(sample_ms, 3),

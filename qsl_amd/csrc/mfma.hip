@@ -52,6 +52,13 @@ struct Swizzle32 {
     }
 };
 
+// Y = U X for the 32-column blocks of a tile.  Per op, every lane loads its
+// A values for all (m-tile, k-step) pairs once (2^(F+1) registers: 64 at
+// F = 5) and the swizzled offsets of the 2^F k-step elements are computed
+// once as uniform values; the swizzle is XOR-linear, so an element at
+// column c (per lane) and k-step offset e (uniform) sits at sw(c) ^ sw(e).
+// Each block then issues all its B reads up front and runs the MT
+// independent accumulator chains back to back.
 template <int F>
 __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const DenseOp &op,
                                             const float *__restrict__ frags)
@@ -69,48 +76,56 @@ __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const Dens
         epos[j] = 1u << op.pos[j];
         tmask |= epos[j];
     }
-    const uint32_t cmask = ((1u << K) - 1) & ~tmask;
+    uint32_t swe[V];
+#pragma unroll
+    for (int ks = 0; ks < V; ks++) {
+        uint32_t eo = 0;
+#pragma unroll
+        for (int j = 0; j < F; j++)
+            if ((ks >> j) & 1) eo |= epos[j];
+        swe[ks] = sw(eo);
+    }
+    float a[MT * V];
     const float *A = frags + op.frag + lane;
+#pragma unroll
+    for (int i = 0; i < MT * V; i++) a[i] = A[i * 64];
+    const uint32_t cmask = ((1u << K) - 1) & ~tmask;
     float *tf = reinterpret_cast<float *>(tile);
     const int hi = lane >> 5, lo = lane & 31;
     for (int cb = wave; cb < nblk; cb += nwaves) {
-        const uint32_t colbase = deposit32((uint32_t)(cb * 32 + lo), cmask);
+        const uint32_t csw = sw(deposit32((uint32_t)(cb * 32 + lo), cmask));
+        float b[V];
+#pragma unroll
+        for (int ks = 0; ks < V; ks++) b[ks] = tf[2 * (csw ^ swe[ks]) + hi];
         floatx16 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
             for (int j = 0; j < 16; j++) acc[mt][j] = 0.f;
-#pragma unroll 4
-        for (int ks = 0; ks < V; ks++) {
-            uint32_t eo = 0;
 #pragma unroll
-            for (int j = 0; j < F; j++)
-                if ((ks >> j) & 1) eo |= epos[j];
-            const float b = tf[2 * sw(colbase | eo) + hi];
+        for (int ks = 0; ks < V; ks++)
 #pragma unroll
-            for (int mt = 0; mt < MT; mt++) {
-                const float a = A[(mt * V + ks) * 64];
-                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[mt], 0, 0, 0);
-            }
-        }
+            for (int mt = 0; mt < MT; mt++)
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt * V + ks], b[ks], acc[mt],
+                                                               0, 0, 0);
 #pragma unroll
         for (int mt = 0; mt < MT; mt++) {
 #pragma unroll
             for (int j = 0; j < 16; j++) {
-                const int R = mt * 32 + 8 * (j >> 2) + 4 * hi + (j & 3);
-                const int r = R >> 1, comp = R & 1;
-                uint32_t eo = 0;
-#pragma unroll
-                for (int q = 0; q < F; q++)
-                    if ((r >> q) & 1) eo |= epos[q];
-                tf[2 * sw(colbase | eo) + comp] = acc[mt][j];
+                // output row R of the embedding: element r = R / 2, part R % 2
+                const int R0 = mt * 32 + 8 * (j >> 2) + (j & 3);  // hi = 0
+                const uint32_t e0 = swe[R0 >> 1], e1 = swe[(R0 + 4) >> 1];
+                // blend without a select (a select here picks between two
+                // addresses of swe and keeps the table out of registers)
+                const uint32_t e = e0 ^ ((e0 ^ e1) & (0u - (uint32_t)hi));
+                tf[2 * (csw ^ e) + (R0 & 1)] = acc[mt][j];
             }
         }
     }
 }
 
 template <int F>
-__global__ __launch_bounds__(kBlock) void k_dense(cplx<float> *__restrict__ st, unsigned n,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_dense(cplx<float> *__restrict__ st, unsigned n,
                                                   int K, int crun, uint64_t tile_mask,
                                                   const DenseOp *__restrict__ ops, int nops,
                                                   const float *__restrict__ frags)
