@@ -118,6 +118,8 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "np_adj_policy") *value = t.np_adj_policy;
         else if (k == "np_stream_min_q") *value = t.np_stream_min_q;
         else if (k == "np_bonds") *value = t.np_bonds;
+        else if (k == "np_lowbonds") *value = t.np_lowbonds;
+        else if (k == "np_hibonds") *value = t.np_hibonds;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
         else if (k == "run_sums_used") *value = g_run_sums_used;
@@ -164,6 +166,12 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "jit") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit must be 0, 1 or 2");
             t.jit = value;
+        } else if (k == "np_hibonds") {
+            QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "np_hibonds must be 0 or 1");
+            t.np_hibonds = value;
+        } else if (k == "np_lowbonds") {
+            QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "np_lowbonds must be 0 or 1");
+            t.np_lowbonds = value;
         } else if (k == "np_bonds") {
             QH_REQUIRE(value >= 1 && value <= 4, QSLHIP_EINVAL, "np_bonds must be in [1, 4]");
             t.np_bonds = value;

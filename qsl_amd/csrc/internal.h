@@ -231,6 +231,10 @@ struct Tuning {
     int reg_min_qubits = 18;     // smaller states: LDS tiles (latency bound there)
     int jit = 1;                 // register-tile passes compiled at run time: 0 never,
                                  // 1 once a pass structure repeats, 2 always
+    int np_lowbonds = 1;         // NP: adjacent disjoint bonds below qubit 12 fused into
+                                 // one pass (0: one pass per bond)
+    int np_hibonds = 1;          // NP: up to 4 disjoint bonds in the high part of the
+                                 // strings fused into one pass (0: one pass per bond)
     int np_bonds = 1;            // NP: swap-like gates on disjoint pairs per pass (1: one
                                  // each; grouping measured slower for config 3)
     int np_adj_max = 4;          // NP: adjacent swap-like gates on qubits q, q+1 with

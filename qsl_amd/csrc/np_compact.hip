@@ -150,8 +150,35 @@ __global__ __launch_bounds__(kBlock) void k_np_gate(cplx<T> *__restrict__ amp, N
     if (CLASS == PC_DIAG2 && ((q0 >= d.L && !(hix & b0)) || (q1 >= d.L && !(hix & b1))))
         return;
     if (CLASS == PC_CTRL && q0 >= d.L && !(hix & b0)) return;
-    if ((CLASS == PC_SWAPLIKE) && q0 >= d.L && q1 >= d.L && !((hix & b0) && !(hix & b1)))
+    if ((CLASS == PC_SWAPLIKE || CLASS == PC_FSWAP) && q0 >= d.L && q1 >= d.L) {
+        // both qubits in the high part: segment hi pairs element i with
+        // element i of segment hi ^ b0 ^ b1 (same weight, same low strings),
+        // two contiguous streams with no per-element table reads
+        const bool own = (hix & b0) && !(hix & b1);
+        if (CLASS == PC_FSWAP && (hix & b0) && (hix & b1)) {
+#pragma unroll
+            for (int e = 0; e < kItems; e++) {
+                const uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+                if (i >= g.size || i >= g.i0 + kChunk) continue;
+                cplx<T> a = amp_ld<S>(amp + g.off + i);
+                one_update<T>(QSLHIP_PAULI_Z, a, c, s);
+                amp_st<S>(amp + g.off + i, a);
+            }
+            return;
+        }
+        if (!own) return;
+        const uint64_t offp = d.seg_off[(hix ^ b0 ^ b1) >> d.L];
+#pragma unroll
+        for (int e = 0; e < kItems; e++) {
+            const uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+            if (i >= g.size || i >= g.i0 + kChunk) continue;
+            cplx<T> a = amp_ld<S>(amp + g.off + i), b = amp_ld<S>(amp + offp + i);
+            pair_update<T>(body, a, b, c, s);
+            amp_st<S>(amp + g.off + i, a);
+            amp_st<S>(amp + offp + i, b);
+        }
         return;
+    }
 #pragma unroll
     for (int e = 0; e < kItems; e++) {
         uint32_t i = g.i0 + e * kBlock + threadIdx.x;
@@ -414,6 +441,143 @@ __global__ __launch_bounds__(kBlock) void k_np_bonds(cplx<T> *__restrict__ amp, 
             if (sub >= na || (sub & ~active)) continue;
             amp[rk[sub]] = v[sub];
         }
+    }
+}
+
+// Adjacent swap-like gates on pairwise disjoint bonds whose qubits are all
+// below kLowBits, in one pass.  Within a segment the strings are in numeric
+// order, so those sharing bits kLowBits and up form a contiguous run (at
+// most C(12, 6) = 924 ranks) that every such gate maps to itself: a bond's
+// partner is r +- C(q, ones below q) (adj_partner) inside the run.  A
+// workgroup takes the runs that start in its chunk, whole (reading up to
+// one run past the chunk's end), stages them in LDS, applies the gates in
+// program order with pair_update (one barrier per gate) and writes them
+// back: each pair sees the same updates in the same order as one launch per
+// gate, so the result is bit-identical.
+constexpr int kLowBits = 12;
+constexpr int kLowBondMax = 8;
+constexpr int kRunMax = 924;  // C(12, 6)
+struct NPLowBonds {
+    int m;
+    int body[kLowBondMax];
+    uint32_t q0[kLowBondMax], q1[kLowBondMax];
+    double c[kLowBondMax], s[kLowBondMax];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_lowbonds(cplx<T> *__restrict__ amp, NPDev d,
+                                                        NPLowBonds B)
+{
+    constexpr int kSpan = kChunk + kRunMax;
+    __shared__ cplx<T> sv[kSpan];
+    __shared__ uint32_t sx[kSpan];
+    __shared__ uint32_t s_first, s_end;
+    const Seg g = seg_of(d);
+    const uint32_t cend = g.i0 + kChunk < g.size ? g.i0 + kChunk : g.size;
+    const uint32_t lim = g.i0 + kSpan < g.size ? g.i0 + kSpan : g.size;
+    if (threadIdx.x == 0) {
+        s_first = cend;  // first run start in [i0, cend)
+        s_end = lim;     // end of the last run that starts before cend
+    }
+    for (uint32_t i = g.i0 + threadIdx.x; i < lim; i += kBlock)
+        sx[i - g.i0] = d.lo_list[g.cstart + i];
+    __syncthreads();
+    // run starts: index i0 starts a run unless the string before it (in the
+    // previous chunk) shares its bits from kLowBits up
+    const uint32_t prev_key = g.i0 > 0 ? (d.lo_list[g.cstart + g.i0 - 1] >> kLowBits) : ~0u;
+    for (uint32_t i = g.i0 + threadIdx.x; i < lim; i += kBlock) {
+        const uint32_t key = sx[i - g.i0] >> kLowBits;
+        const uint32_t before = i == g.i0 ? prev_key : (sx[i - 1 - g.i0] >> kLowBits);
+        if (key != before) {
+            if (i < cend) atomicMin(&s_first, i);
+            else atomicMin(&s_end, i);
+        }
+    }
+    __syncthreads();
+    const uint32_t first = s_first, end = s_first < cend ? s_end : s_first;
+    for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) sv[i - g.i0] = amp[g.off + i];
+    __syncthreads();
+    for (int j = 0; j < B.m; j++) {
+        const unsigned q0 = B.q0[j], q1 = B.q1[j], q = q0 < q1 ? q0 : q1;
+        const T c = (T)B.c[j], s = (T)B.s[j];
+        for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
+            const uint32_t x = sx[i - g.i0];
+            if (!((x >> q0) & 1u) || ((x >> q1) & 1u)) continue;
+            const uint64_t p = adj_partner(d, x, i, q0, q1);
+            cplx<T> a = sv[i - g.i0], b = sv[p - g.i0];
+            pair_update<T>(B.body[j], a, b, c, s);
+            sv[i - g.i0] = a;
+            sv[p - g.i0] = b;
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) amp[g.off + i] = sv[i - g.i0];
+}
+
+// Up to kBondMax swap-like gates on pairwise disjoint qubit pairs that all
+// lie in the high part of the strings, in one pass.  Such a gate maps
+// segment hi onto segment hi ^ b0 ^ b1 element for element, so the segments
+// reachable from hi by exchanging the bits of its active pairs (the two
+// bits differ) form an orbit of 2^a segments of the same weight.  The
+// blocks of the orbit's representative (every active pair holds bit q0 = 1,
+// q1 = 0) gather element i of each member, apply the gates in program
+// order with pair_update and scatter them back: the same updates in the same
+// order as one launch per gate, so the result is bit-identical.  Other
+// blocks return at once.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_hibonds(cplx<T> *__restrict__ amp, NPDev d,
+                                                       NPBonds B)
+{
+    const Seg g = seg_of(d);
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+    unsigned active = 0;
+#pragma unroll
+    for (int j = 0; j < kBondMax; j++) {
+        if (j >= B.m) break;
+        const bool a0 = (hix >> B.q0[j]) & 1ull, a1 = (hix >> B.q1[j]) & 1ull;
+        if (a0 != a1) {
+            if (!a0) return;  // another member of the orbit represents it
+            active |= 1u << j;
+        }
+    }
+    if (!active) return;
+    uint64_t off[1 << kBondMax];
+#pragma unroll
+    for (int sub = 0; sub < (1 << kBondMax); sub++) {
+        off[sub] = 0;
+        if (sub & ~active) continue;
+        uint64_t y = hix;
+#pragma unroll
+        for (int j = 0; j < kBondMax; j++)
+            if ((sub >> j) & 1) y ^= (1ull << B.q0[j]) | (1ull << B.q1[j]);
+        off[sub] = sub ? d.seg_off[y >> d.L] : g.off;
+    }
+#pragma unroll 1
+    for (int e = 0; e < kItems; e++) {
+        const uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        if (i >= g.size || i >= g.i0 + kChunk) continue;
+        cplx<T> v[1 << kBondMax];
+#pragma unroll
+        for (int sub = 0; sub < (1 << kBondMax); sub++)
+            if (!(sub & ~active)) v[sub] = amp[off[sub] + i];
+#pragma unroll
+        for (int j = 0; j < kBondMax; j++) {
+            if (j >= B.m || !((active >> j) & 1)) continue;
+            const T c = (T)B.c[j], s = (T)B.s[j];
+#pragma unroll
+            for (int sub = 0; sub < (1 << kBondMax); sub++) {
+                if ((sub & ~active) || ((sub >> j) & 1)) continue;
+                // v[sub] holds q0 = 1, q1 = 0 for pair j; its partner is sub | bit j
+                const int par = sub | (1 << j);
+                cplx<T> a = v[sub], b = v[par];
+                pair_update<T>(B.body[j], a, b, c, s);
+                v[sub] = a;
+                v[par] = b;
+            }
+        }
+#pragma unroll
+        for (int sub = 0; sub < (1 << kBondMax); sub++)
+            if (!(sub & ~active)) amp[off[sub] + i] = v[sub];
     }
 }
 
@@ -777,7 +941,7 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
     // k_np_diag pass each, their ops staged in one copy), groups of two to
     // kBondMax swap-like gates on disjoint qubit pairs (one k_np_bonds pass
     // each), and single gates.
-    enum { SINGLE, DIAG, BONDS };
+    enum { SINGLE, DIAG, BONDS, LOWBONDS, HIBONDS };
     struct Part { int type; size_t first, end; };
     std::vector<Part> parts;
     for (size_t i = 0; i < gates.size();) {
@@ -789,6 +953,43 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
             continue;
         }
         uint64_t used = 0;
+        // adjacent bonds below kLowBits (and inside the low part of the
+        // strings), pairwise disjoint: one k_np_lowbonds pass
+        const unsigned lowq = h->npt->L < (unsigned)kLowBits ? h->npt->L : (unsigned)kLowBits;
+        j = i;
+        while (tuning().np_lowbonds && j < gates.size() && j - i < (size_t)kLowBondMax &&
+               bond_kind(gates[j].kind)) {
+            const GateDesc &b = gates[j];
+            const unsigned lo = b.q0 < b.q1 ? b.q0 : b.q1, hi = b.q0 < b.q1 ? b.q1 : b.q0;
+            if (hi != lo + 1 || hi >= lowq) break;
+            const uint64_t m = (1ull << b.q0) | (1ull << b.q1);
+            if (used & m) break;
+            used |= m;
+            j++;
+        }
+        if (j - i >= 2) {
+            parts.push_back({LOWBONDS, i, j});
+            i = j;
+            continue;
+        }
+        // swap-like gates on disjoint pairs in the high part: one
+        // k_np_hibonds pass per group of up to kBondMax
+        used = 0;
+        j = i;
+        while (tuning().np_hibonds && j < gates.size() && j - i < (size_t)kBondMax &&
+               bond_kind(gates[j].kind) && gates[j].q0 >= h->npt->L &&
+               gates[j].q1 >= h->npt->L) {
+            const uint64_t m = (1ull << gates[j].q0) | (1ull << gates[j].q1);
+            if (used & m) break;
+            used |= m;
+            j++;
+        }
+        if (j - i >= 2) {
+            parts.push_back({HIBONDS, i, j});
+            i = j;
+            continue;
+        }
+        used = 0;
         j = i;
         const size_t gmax = (size_t)tuning().np_bonds;
         while (j < gates.size() && j - i < gmax && bond_kind(gates[j].kind)) {
@@ -849,8 +1050,8 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
                 hipLaunchKernelGGL(k_np_diag<float>, grid, blk, 0, h->stream,
                                    static_cast<cplx<float> *>(h->amps), d, dev + first_op, nops);
             first_op += (size_t)nops;
-        } else {
-            NPBonds B{};
+        } else if (pt.type == LOWBONDS) {
+            NPLowBonds B{};
             B.m = (int)(pt.end - pt.first);
             for (int j = 0; j < B.m; j++) {
                 const GateDesc &g = gates[pt.first + j];
@@ -861,11 +1062,36 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
                 B.s[j] = g.s;
             }
             if (h->prec == 8)
-                hipLaunchKernelGGL(k_np_bonds<double>, grid, blk, 0, h->stream,
+                hipLaunchKernelGGL(k_np_lowbonds<double>, grid, blk, 0, h->stream,
                                    static_cast<cplx<double> *>(h->amps), d, B);
             else
+                hipLaunchKernelGGL(k_np_lowbonds<float>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<float> *>(h->amps), d, B);
+        } else {  // BONDS, HIBONDS
+            NPBonds B{};
+            B.m = (int)(pt.end - pt.first);
+            for (int j = 0; j < B.m; j++) {
+                const GateDesc &g = gates[pt.first + j];
+                B.body[j] = body_of(g.kind);
+                B.q0[j] = g.q0;
+                B.q1[j] = g.q1;
+                B.c[j] = g.c;
+                B.s[j] = g.s;
+            }
+            if (pt.type == HIBONDS) {
+                if (h->prec == 8)
+                    hipLaunchKernelGGL(k_np_hibonds<double>, grid, blk, 0, h->stream,
+                                       static_cast<cplx<double> *>(h->amps), d, B);
+                else
+                    hipLaunchKernelGGL(k_np_hibonds<float>, grid, blk, 0, h->stream,
+                                       static_cast<cplx<float> *>(h->amps), d, B);
+            } else if (h->prec == 8) {
+                hipLaunchKernelGGL(k_np_bonds<double>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<double> *>(h->amps), d, B);
+            } else {
                 hipLaunchKernelGGL(k_np_bonds<float>, grid, blk, 0, h->stream,
                                    static_cast<cplx<float> *>(h->amps), d, B);
+            }
         }
         QH_CHECK(hipGetLastError());
     }

@@ -263,3 +263,42 @@ def test_np_disjoint_bond_groups_in_one_pass(bond_groups, dtype, n, k):
             else:
                 O.np_gate1(want, k, kind, a, ang)
         assert _bits_equal(fused.getState(), want)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,k", [(14, 6), (20, 10), (22, 4), (26, 13)])
+def test_np_fused_low_and_high_bond_passes(dtype, n, k):
+    # groups of swap-like gates on disjoint adjacent pairs below qubit 12
+    # (one pass over runs of strings that share the bits above, staged in
+    # LDS) and groups of up to four on disjoint pairs in the high part of
+    # the strings (one pass over orbits of segments): bit-identical to one
+    # kernel per gate, for every swap-like kind and both orientations
+    rng = np.random.default_rng(900 + n + k)
+    L = min(n, max(16, n - 16))
+    seq = []
+    for rep in range(12):
+        lows = list(range(rep % 2, min(L, 12) - 1, 2))
+        for q in rng.permutation(lows)[: int(rng.integers(2, 7))]:
+            a, b = (int(q), int(q) + 1) if rng.random() < 0.5 else (int(q) + 1, int(q))
+            seq.append((int(rng.choice([24, 26, 27, 28])), a, b, 0, _angle(rng, dtype)))
+        if n - L >= 4:
+            hs = rng.permutation(np.arange(L, n))
+            for j in range(min(4, (n - L) // 2)):
+                seq.append((int(rng.choice([24, 26, 27, 28])), int(hs[2 * j]), int(hs[2 * j + 1]),
+                            0, _angle(rng, dtype)))
+        seq.append((4, int(rng.integers(n)), 0, 0, _angle(rng, dtype)))
+    ops = np.array(seq, OP_DTYPE)
+    total = math.comb(n, k)
+    amp = (rng.normal(size=total) + 1j * rng.normal(size=total)).astype(dtype)
+    amp /= np.linalg.norm(amp)
+    fused = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    per_gate = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    fused.setAmps(amp)
+    per_gate.setAmps(amp)
+    fused.apply_ops(ops)
+    per_gate.apply_ops(ops, mode=qsl_amd.MODE_PER_GATE)
+    assert _bits_equal(fused.getAmps(), per_gate.getAmps())
+    if dtype == np.complex128 and n <= 22:
+        want = amp.astype(np.complex128)
+        O.npc_apply_ops(want, n, k, ops, nthreads=8)
+        assert _bits_equal(fused.getAmps(), want)
