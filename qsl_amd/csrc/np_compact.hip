@@ -495,15 +495,35 @@ __global__ __launch_bounds__(kBlock) void k_np_lowbonds(cplx<T> *__restrict__ am
     }
     __syncthreads();
     const uint32_t first = s_first, end = s_first < cend ? s_end : s_first;
-    for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) sv[i - g.i0] = amp[g.off + i];
+    // C(q, m) for q < kLowBits in LDS (the partner offsets), and each
+    // thread's strings in registers for every gate
+    __shared__ uint32_t sbin[kLowBits][kLowBits + 1];
+    for (int t = threadIdx.x; t < kLowBits * (kLowBits + 1); t += kBlock)
+        sbin[t / (kLowBits + 1)][t % (kLowBits + 1)] =
+            (uint32_t)d.binom[(t / (kLowBits + 1)) * 64 + t % (kLowBits + 1)];
+    constexpr int kPer = (kSpan + kBlock - 1) / kBlock;
+    uint32_t xs[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+        const uint32_t i = first + threadIdx.x + e * kBlock;
+        xs[e] = 0;
+        if (i < end) {
+            sv[i - g.i0] = amp[g.off + i];
+            xs[e] = sx[i - g.i0];
+        }
+    }
     __syncthreads();
     for (int j = 0; j < B.m; j++) {
         const unsigned q0 = B.q0[j], q1 = B.q1[j], q = q0 < q1 ? q0 : q1;
         const T c = (T)B.c[j], s = (T)B.s[j];
-        for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) {
-            const uint32_t x = sx[i - g.i0];
-            if (!((x >> q0) & 1u) || ((x >> q1) & 1u)) continue;
-            const uint64_t p = adj_partner(d, x, i, q0, q1);
+#pragma unroll
+        for (int e = 0; e < kPer; e++) {
+            const uint32_t i = first + threadIdx.x + e * kBlock;
+            const uint32_t x = xs[e];
+            // owners: bit q0 set, q1 clear (strings past `end` are 0: never owners)
+            if (i >= end || !((x >> q0) & 1u) || ((x >> q1) & 1u)) continue;
+            const uint32_t m = sbin[q][__popc(x & ((1u << q) - 1))];
+            const uint32_t p = q1 > q0 ? i + m : i - m;
             cplx<T> a = sv[i - g.i0], b = sv[p - g.i0];
             pair_update<T>(B.body[j], a, b, c, s);
             sv[i - g.i0] = a;
@@ -511,7 +531,11 @@ __global__ __launch_bounds__(kBlock) void k_np_lowbonds(cplx<T> *__restrict__ am
         }
         __syncthreads();
     }
-    for (uint32_t i = first + threadIdx.x; i < end; i += kBlock) amp[g.off + i] = sv[i - g.i0];
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+        const uint32_t i = first + threadIdx.x + e * kBlock;
+        if (i < end) amp[g.off + i] = sv[i - g.i0];
+    }
 }
 
 // Up to kBondMax swap-like gates on pairwise disjoint qubit pairs that all
