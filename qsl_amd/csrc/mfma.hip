@@ -57,8 +57,8 @@ struct Swizzle32 {
 // F = 5) and the swizzled offsets of the 2^F k-step elements are computed
 // once as uniform values; the swizzle is XOR-linear, so an element at
 // column c (per lane) and k-step offset e (uniform) sits at sw(c) ^ sw(e).
-// Each block then issues all its B reads up front and runs the MT
-// independent accumulator chains back to back.
+// Each block then runs the MT independent accumulator chains back to back,
+// one LDS read of B per k-step (scheduled ahead by the compiler).
 template <int F>
 __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const DenseOp &op,
                                             const float *__restrict__ frags)
@@ -94,20 +94,19 @@ __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const Dens
     const int hi = lane >> 5, lo = lane & 31;
     for (int cb = wave; cb < nblk; cb += nwaves) {
         const uint32_t csw = sw(deposit32((uint32_t)(cb * 32 + lo), cmask));
-        float b[V];
-#pragma unroll
-        for (int ks = 0; ks < V; ks++) b[ks] = tf[2 * (csw ^ swe[ks]) + hi];
         floatx16 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
             for (int j = 0; j < 16; j++) acc[mt][j] = 0.f;
 #pragma unroll
-        for (int ks = 0; ks < V; ks++)
+        for (int ks = 0; ks < V; ks++) {
+            const float b = tf[2 * (csw ^ swe[ks]) + hi];
 #pragma unroll
             for (int mt = 0; mt < MT; mt++)
-                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt * V + ks], b[ks], acc[mt],
-                                                               0, 0, 0);
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[mt * V + ks], b, acc[mt], 0, 0,
+                                                               0);
+        }
 #pragma unroll
         for (int mt = 0; mt < MT; mt++) {
 #pragma unroll
@@ -125,7 +124,7 @@ __device__ __forceinline__ void dense_apply(cplx<float> *tile, int K, const Dens
 }
 
 template <int F>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_dense(cplx<float> *__restrict__ st, unsigned n,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dense(cplx<float> *__restrict__ st, unsigned n,
                                                   int K, int crun, uint64_t tile_mask,
                                                   const DenseOp *__restrict__ ops, int nops,
                                                   const float *__restrict__ frags)
@@ -140,10 +139,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const int hib = K - crun;
     for (int m = threadIdx.x; m < (1 << hib); m += kBlock) hoff[m] = deposit(m, hi_mask);
     __syncthreads();
-    const uint32_t nt = 1u << K;
+    // the tile is 2^kDenseTile amplitudes (K == kDenseTile): every thread
+    // issues its NE loads before the first wait, then fills LDS
+    // (in two halves of NE / 2: fewer live registers)
+    constexpr int NE = (1 << kDenseTile) / kBlock, NH = NE / 2;
     const uint32_t lowm = (1u << crun) - 1;
-    for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        tile[sw(e)] = st[gbase | (e & lowm) | hoff[e >> crun]];
+    for (int h = 0; h < 2; h++) {
+        cplx<float> v[NH];
+#pragma unroll
+        for (int k = 0; k < NH; k++) {
+            const uint32_t e = threadIdx.x + (h * NH + k) * kBlock;
+            v[k] = st[gbase | (e & lowm) | hoff[e >> crun]];
+        }
+#pragma unroll
+        for (int k = 0; k < NH; k++) tile[sw(threadIdx.x + (h * NH + k) * kBlock)] = v[k];
+    }
     __syncthreads();
     for (int i = 0; i < nops; i++) {
         const DenseOp op = ops[i];
@@ -151,8 +161,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         else dense_apply<F>(tile, K, op, frags);
         __syncthreads();
     }
-    for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        st[gbase | (e & lowm) | hoff[e >> crun]] = tile[sw(e)];
+    // the store addresses are recomputed here, not kept in registers from
+    // the loads across the op loop (where they would spill): the empty asm
+    // statements make the thread index and crun opaque values, so nothing
+    // derived from them before the loop is reused
+    uint32_t tid = threadIdx.x, crun_s = (uint32_t)crun;
+    asm volatile("" : "+v"(tid));
+    asm volatile("" : "+s"(crun_s));
+    const uint32_t lowm_s = (1u << crun_s) - 1;
+    for (int h = 0; h < 2; h++) {
+        cplx<float> v[NH];
+#pragma unroll
+        for (int k = 0; k < NH; k++) v[k] = tile[sw(tid + (h * NH + k) * kBlock)];
+#pragma unroll
+        for (int k = 0; k < NH; k++) {
+            const uint32_t e = tid + (h * NH + k) * kBlock;
+            st[gbase | (e & lowm_s) | hoff[e >> crun_s]] = v[k];
+        }
+    }
 }
 
 }  // namespace
