@@ -22,6 +22,50 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxHiBits = 8;
 
+// Group <-> LDS tile copies, eight elements per thread at a time: all
+// eight loads are issued before the first wait (a loop of single
+// load-then-store keeps one load in flight per thread).
+template <bool S, typename T>
+__device__ __forceinline__ void tile_load(cplx<T> *tile, const cplx<T> *st, uint32_t nt,
+                                          uint64_t gbase, uint32_t lowm, int crun,
+                                          const uint64_t *hoff)
+{
+    for (uint32_t e0 = threadIdx.x; e0 < nt; e0 += 8 * kBlock) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            // past the tile (K < 11): reload the first element, not stored
+            const uint32_t e = e0 + k * kBlock < nt ? e0 + k * kBlock : e0;
+            v[k] = amp_ld<S>(st + (gbase | (e & lowm) | hoff[e >> crun]));
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t e = e0 + k * kBlock;
+            if (e < nt) tile[e] = v[k];
+        }
+    }
+}
+
+template <bool S, typename T>
+__device__ __forceinline__ void tile_store(const cplx<T> *tile, cplx<T> *st, uint32_t nt,
+                                           uint64_t gbase, uint32_t lowm, int crun,
+                                           const uint64_t *hoff)
+{
+    for (uint32_t e0 = threadIdx.x; e0 < nt; e0 += 8 * kBlock) {
+        cplx<T> v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t e = e0 + k * kBlock < nt ? e0 + k * kBlock : e0;
+            v[k] = tile[e];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t e = e0 + k * kBlock;
+            if (e < nt) amp_st<S>(st + (gbase | (e & lowm) | hoff[e >> crun]), v[k]);
+        }
+    }
+}
+
 // One workgroup per group of 2^K amplitudes.
 //   tile_mask: the K tile qubits; crun: length of its low contiguous run
 //   (qubits 0..crun-1); the other K-crun <= 8 tile qubits are addressed
@@ -46,8 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(cplx<T> *__restrict__ st,
 
     const uint32_t nt = 1u << K;
     const uint32_t lowm = (1u << crun) - 1;
-    for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        tile[e] = st[gbase | (e & lowm) | hoff[e >> crun]];
+    tile_load<false>(tile, st, nt, gbase, lowm, crun, hoff);
     __syncthreads();
 
     for (int i = 0; i < nops; i++) {
@@ -56,8 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(cplx<T> *__restrict__ st,
         __syncthreads();
     }
 
-    for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        st[gbase | (e & lowm) | hoff[e >> crun]] = tile[e];
+    tile_store<false>(tile, st, nt, gbase, lowm, crun, hoff);
 }
 
 // The same pass with one op passed by value (the per-gate path for low
@@ -78,13 +120,11 @@ __global__ __launch_bounds__(kBlock) void k_tile1(cplx<T> *__restrict__ st, unsi
     __syncthreads();
     const uint32_t nt = 1u << K;
     const uint32_t lowm = (1u << crun) - 1;
-    for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        tile[e] = amp_ld<S>(st + (gbase | (e & lowm) | hoff[e >> crun]));
+    tile_load<S>(tile, st, nt, gbase, lowm, crun, hoff);
     __syncthreads();
     tile_op<T>(tile, K, gbase, op);
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < nt; e += kBlock)
-        amp_st<S>(st + (gbase | (e & lowm) | hoff[e >> crun]), tile[e]);
+    tile_store<S>(tile, st, nt, gbase, lowm, crun, hoff);
 }
 
 template <typename T>

@@ -47,21 +47,38 @@ __device__ __forceinline__ cplx<T> scaled(cplx<T> a, double factor)
 
 // Source amplitudes [first, first + count): block m = i >> targ goes to
 // block 2m, and block 2m + 1 is zeroed.
+// Both kernels take kUnroll amplitudes per thread per step, all loads
+// issued before the stores (a load-then-store loop keeps one load in flight
+// per thread).  Within a launch the read and written ranges are disjoint
+// (or the same index), so the order inside a step does not matter.
+constexpr int kUnroll = 4;
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_expand(const cplx<T> *src, cplx<T> *dst,
                                                    uint64_t first, uint64_t count, unsigned targ)
 {
     const uint64_t lomask = (1ull << targ) - 1;
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < count; f += stride) {
-        const uint64_t i = first + f;
-        const uint64_t to = ((i & ~lomask) << 1) | (i & lomask);
-        const cplx<T> a = src[i];
-        cplx<T> z;
-        z.re = 0;
-        z.im = 0;
-        dst[to] = a;
-        dst[to | (lomask + 1)] = z;
+    for (uint64_t f0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f0 < count;
+         f0 += kUnroll * stride) {
+        cplx<T> a[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint64_t f = f0 + u * stride;
+            a[u] = src[first + (f < count ? f : f0)];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint64_t f = f0 + u * stride;
+            if (f >= count) continue;
+            const uint64_t i = first + f;
+            const uint64_t to = ((i & ~lomask) << 1) | (i & lomask);
+            cplx<T> z;
+            z.re = 0;
+            z.im = 0;
+            dst[to] = a[u];
+            dst[to | (lomask + 1)] = z;
+        }
     }
 }
 
@@ -75,10 +92,20 @@ __global__ __launch_bounds__(kBlock) void k_compact(const cplx<T> *src, cplx<T> 
     const uint64_t lomask = (1ull << targ) - 1;
     const uint64_t ob = (uint64_t)outcome << targ;
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t f = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f < count; f += stride) {
-        const uint64_t j = first + f;
-        const uint64_t from = ((j & ~lomask) << 1) | ob | (j & lomask);
-        dst[j] = scaled(src[from], factor);
+    for (uint64_t f0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x; f0 < count;
+         f0 += kUnroll * stride) {
+        cplx<T> a[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint64_t f = f0 + u * stride;
+            const uint64_t j = first + (f < count ? f : f0);
+            a[u] = src[((j & ~lomask) << 1) | ob | (j & lomask)];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint64_t f = f0 + u * stride;
+            if (f < count) dst[first + f] = scaled(a[u], factor);
+        }
     }
 }
 
