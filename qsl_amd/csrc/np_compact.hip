@@ -469,25 +469,24 @@ __global__ __launch_bounds__(kBlock) void k_np_lowbonds(cplx<T> *__restrict__ am
                                                         NPLowBonds B)
 {
     constexpr int kSpan = kChunk + kRunMax;
+    // (the strings come from the table, L2-resident, not from LDS: 31.8 KiB
+    // per workgroup for FP64 leaves room for five per CU)
     __shared__ cplx<T> sv[kSpan];
-    __shared__ uint32_t sx[kSpan];
     __shared__ uint32_t s_first, s_end;
     const Seg g = seg_of(d);
+    const uint32_t *lo = d.lo_list + g.cstart;
     const uint32_t cend = g.i0 + kChunk < g.size ? g.i0 + kChunk : g.size;
     const uint32_t lim = g.i0 + kSpan < g.size ? g.i0 + kSpan : g.size;
     if (threadIdx.x == 0) {
         s_first = cend;  // first run start in [i0, cend)
         s_end = lim;     // end of the last run that starts before cend
     }
-    for (uint32_t i = g.i0 + threadIdx.x; i < lim; i += kBlock)
-        sx[i - g.i0] = d.lo_list[g.cstart + i];
     __syncthreads();
-    // run starts: index i0 starts a run unless the string before it (in the
-    // previous chunk) shares its bits from kLowBits up
-    const uint32_t prev_key = g.i0 > 0 ? (d.lo_list[g.cstart + g.i0 - 1] >> kLowBits) : ~0u;
+    // run starts: index i starts a run unless the string before it shares
+    // its bits from kLowBits up
     for (uint32_t i = g.i0 + threadIdx.x; i < lim; i += kBlock) {
-        const uint32_t key = sx[i - g.i0] >> kLowBits;
-        const uint32_t before = i == g.i0 ? prev_key : (sx[i - 1 - g.i0] >> kLowBits);
+        const uint32_t key = lo[i] >> kLowBits;
+        const uint32_t before = i > 0 ? (lo[i - 1] >> kLowBits) : ~0u;
         if (key != before) {
             if (i < cend) atomicMin(&s_first, i);
             else atomicMin(&s_end, i);
@@ -509,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_np_lowbonds(cplx<T> *__restrict__ am
         xs[e] = 0;
         if (i < end) {
             sv[i - g.i0] = amp[g.off + i];
-            xs[e] = sx[i - g.i0];
+            xs[e] = lo[i];
         }
     }
     __syncthreads();
