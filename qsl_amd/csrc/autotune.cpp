@@ -119,6 +119,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "np_stream_min_q") *value = t.np_stream_min_q;
         else if (k == "np_bonds") *value = t.np_bonds;
         else if (k == "np_lowbonds") *value = t.np_lowbonds;
+        else if (k == "pass_stream") *value = t.pass_stream;
         else if (k == "np_hibonds") *value = t.np_hibonds;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
@@ -166,6 +167,9 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "jit") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit must be 0, 1 or 2");
             t.jit = value;
+        } else if (k == "pass_stream") {
+            QH_REQUIRE(value >= 0 && value <= 3, QSLHIP_EINVAL, "pass_stream must be in [0, 3]");
+            t.pass_stream = value;
         } else if (k == "np_hibonds") {
             QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "np_hibonds must be 0 or 1");
             t.np_hibonds = value;

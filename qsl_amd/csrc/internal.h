@@ -246,6 +246,8 @@ struct Tuning {
     int np_stream_min_q = 64;    // NP gates whose lowest qubit is at least this use
                                  // streaming accesses (states of stream_min_qubits+);
                                  // off: on C(32,16) FP64 they run 5-10% slower
+    int pass_stream = 3;         // register-tile passes on such states: streaming
+                                 // loads (bit 0) and stores (bit 1)
     int lane_gates = 1;          // one-qubit gates on targets 0..5 by cross-lane
                                  // exchange (0: one-op LDS tiles / streaming pairs)
     int stream_min_qubits = 26;  // states of at least this many qubits (1 GiB FP64)

@@ -361,7 +361,7 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
     ta.gfix = 0;
     ta.zmask = 0;
     ta.zval = 0;
-    ta.stream = n >= (unsigned)tuning().stream_min_qubits;
+    ta.stream = n >= (unsigned)tuning().stream_min_qubits ? (uint32_t)tuning().pass_stream : 0u;
     int pos = 0;
     for (unsigned q = 0; q < 64; q++) {
         if (!((p.tile_mask >> q) & 1ull)) continue;

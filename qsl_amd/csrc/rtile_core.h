@@ -44,8 +44,8 @@ struct RegTileArgs {
                                  // pass over every group
     uint64_t zmask, zval;        // amplitudes whose bits under zmask differ
                                  // from zval are zero and not in memory yet
-    uint32_t stream;             // nonzero: streaming (nontemporal) loads and
-                                 // stores, for states far larger than the caches
+    uint32_t stream;             // streaming (nontemporal) accesses, for states far
+                                 // larger than the caches: bit 0 loads, bit 1 stores
 };
 
 // The sampling sums of the amplitudes about to be stored, in the pass's
@@ -88,13 +88,13 @@ __device__ __forceinline__ void reg_load(cplx<T> (&v)[R], const cplx<T> *st, uin
         for (int k = 0; k < R; k++) {
             const uint64_t i = idx0 | kpart[k];
             if ((i & ta.zmask) == ta.zval) {
-                v[k] = ta.stream ? amp_ld<true>(st + i) : amp_ld<false>(st + i);
+                v[k] = (ta.stream & 1u) ? amp_ld<true>(st + i) : amp_ld<false>(st + i);
             } else {
                 v[k].re = T(0);
                 v[k].im = T(0);
             }
         }
-    } else if (ta.stream) {
+    } else if (ta.stream & 1u) {
 #pragma unroll
         for (int k = 0; k < R; k++) v[k] = amp_ld<true>(st + (idx0 | kpart[k]));
     } else {
@@ -108,7 +108,7 @@ template <typename T, int R>
 __device__ __forceinline__ void reg_store(const cplx<T> (&v)[R], cplx<T> *st, uint64_t idx0,
                                           const uint64_t (&kpart)[R], const RegTileArgs &ta)
 {
-    if (ta.stream) {
+    if (ta.stream & 2u) {
 #pragma unroll
         for (int k = 0; k < R; k++) amp_st<true>(st + (idx0 | kpart[k]), v[k]);
     } else {
