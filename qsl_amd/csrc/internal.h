@@ -235,6 +235,8 @@ struct Tuning {
                                  // one pass (0: one pass per bond)
     int np_hibonds = 1;          // NP: up to 4 disjoint bonds in the high part of the
                                  // strings fused into one pass (0: one pass per bond)
+    int np_phase_lut = 1;        // NP: runs of phase gates through per-string masks of the
+                                 // gates acting on it (0: every gate tested per amplitude)
     int np_bonds = 1;            // NP: swap-like gates on disjoint pairs per pass (1: one
                                  // each; grouping measured slower for config 3)
     int np_adj_max = 4;          // NP: adjacent swap-like gates on qubits q, q+1 with

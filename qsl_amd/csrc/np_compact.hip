@@ -341,6 +341,13 @@ __global__ __launch_bounds__(kBlock) void k_np_diag(cplx<T> *__restrict__ amp, N
         const T c = (T)o.c, s = (T)o.s;
         switch (o.cls) {
         case PC_ONE:
+            if (o.q0 >= d.L) {  // a bit of the segment: the same for every item
+                if ((hix >> o.q0) & 1ull) {
+#pragma unroll
+                    for (int e = 0; e < kItems; e++) one_update<T>(o.body, a[e], c, s);
+                }
+                break;
+            }
 #pragma unroll
             for (int e = 0; e < kItems; e++)
                 if ((x[e] >> o.q0) & 1ull) one_update<T>(o.body, a[e], c, s);
@@ -361,6 +368,126 @@ __global__ __launch_bounds__(kBlock) void k_np_diag(cplx<T> *__restrict__ amp, N
                 if ((x[e] >> o.q0) & 1ull) rz_half<T>((x[e] >> o.q1) & 1ull, a[e], c, s);
             break;
         }
+    }
+#pragma unroll
+    for (int e = 0; e < kItems; e++)
+        if (in[e]) amp[g.off + g.i0 + e * kBlock + threadIdx.x] = a[e];
+}
+
+// A run of up to kPhaseMax phase gates (np/gates.cpp:33-81) in one pass,
+// taken in program order.  A gate on a qubit >= L tests a bit of the
+// segment, the same for every string of the workgroup: it is applied to all
+// of them or skipped as a whole.  A maximal stretch of gates on low qubits
+// changes each amplitude only by the gates whose qubit is 1 in its string:
+// each thread looks up, per string, the mask of those run indices (one table
+// word per byte of the low part: lut[b][v] holds the gates on qubit 8b + i
+// for each bit i set in v) and iterates over its set bits, lowest first.
+// Strings of one segment have the same number of low ones, so the lanes
+// iterate in lockstep.  Every acting gate is applied with one_update's
+// arithmetic in program order: bit-identical to one launch per gate.
+constexpr int kPhaseMax = 32;
+constexpr int kPhaseBytes = 3;  // low parts of up to 24 bits
+struct NPPhaseRun {
+    int nops;                // gates in the run, at most kPhaseMax
+    int nbytes;              // bytes of the low part the table covers
+    uint32_t lowmask;        // run indices of the gates on qubits below L
+    uint32_t q[kPhaseMax];   // qubit of each gate
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_phases(cplx<T> *__restrict__ amp, NPDev d,
+                                                      const uint32_t *__restrict__ lut,
+                                                      const double *__restrict__ cs,
+                                                      NPPhaseRun run)
+{
+    __shared__ uint32_t s_lut[kPhaseBytes * 256];
+    __shared__ cplx<T> s_cs[kPhaseMax];
+    for (int i = threadIdx.x; i < run.nbytes * 256; i += kBlock) s_lut[i] = lut[i];
+    if (threadIdx.x < (unsigned)run.nops) {
+        s_cs[threadIdx.x].re = (T)cs[2 * threadIdx.x];
+        s_cs[threadIdx.x].im = (T)cs[2 * threadIdx.x + 1];
+    }
+    const Seg g = seg_of(d);
+    uint32_t lo[kItems];
+    cplx<T> a[kItems];
+    bool in[kItems];
+#pragma unroll
+    for (int e = 0; e < kItems; e++) {
+        const uint32_t i = g.i0 + e * kBlock + threadIdx.x;
+        in[e] = i < g.size && i < g.i0 + kChunk;
+        lo[e] = 0;
+        a[e].re = a[e].im = (T)0;
+        if (in[e]) {
+            lo[e] = d.lo_list[g.cstart + i];
+            a[e] = amp[g.off + i];
+        }
+    }
+    __syncthreads();
+    uint32_t m[kItems];  // the low gates acting on each string
+#pragma unroll
+    for (int e = 0; e < kItems; e++) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int b = 0; b < kPhaseBytes; b++)
+            if (b < run.nbytes) w |= s_lut[b * 256 + ((lo[e] >> (8 * b)) & 255u)];
+        m[e] = w;
+    }
+    const uint32_t nops = (uint32_t)run.nops;
+    for (uint32_t j = 0; j < nops;) {
+        if (!((run.lowmask >> j) & 1u)) {  // a segment bit: uniform
+            const uint32_t q = run.q[j] - d.L;
+            if ((g.hi >> q) & 1u) {
+                const T c = (T)cs[2 * j], s = (T)cs[2 * j + 1];
+#pragma unroll
+                for (int e = 0; e < kItems; e++) one_update<T>(QSLHIP_PHASE, a[e], c, s);
+            }
+            j++;
+            continue;
+        }
+        // the stretch of low gates [j, end)
+        const uint32_t high_after = ~run.lowmask & (~0u << j);
+        const uint32_t end = high_after ? (uint32_t)(__ffs(high_after) - 1) : nops;
+        const uint32_t span = (end >= 32 ? ~0u : ((1u << end) - 1)) & (~0u << j);
+        uint32_t mm[kItems];
+#pragma unroll
+        for (int e = 0; e < kItems; e++) mm[e] = m[e] & span;
+        // the usual case (a stretch over distinct qubits): every string of
+        // the wavefront has the same number of acting gates, and the loop
+        // needs no per-lane selects
+        const uint32_t cnt = __builtin_amdgcn_readfirstlane(__popc(mm[0]));
+        bool same = true;
+#pragma unroll
+        for (int e = 0; e < kItems; e++) same = same && (uint32_t)__popc(mm[e]) == cnt;
+        if (__all(same)) {
+            for (uint32_t it = 0; it < cnt; it++) {
+                cplx<T> f[kItems];
+#pragma unroll
+                for (int e = 0; e < kItems; e++)  // cnt more set bits: mm[e] != 0
+                    f[e] = s_cs[__builtin_ctz(mm[e])];
+#pragma unroll
+                for (int e = 0; e < kItems; e++) {
+                    one_update<T>(QSLHIP_PHASE, a[e], f[e].re, f[e].im);
+                    mm[e] &= mm[e] - 1;
+                }
+            }
+            j = end;
+            continue;
+        }
+        while (mm[0] | mm[1] | mm[2] | mm[3]) {
+            cplx<T> f[kItems];
+#pragma unroll
+            for (int e = 0; e < kItems; e++) f[e] = s_cs[(__ffs(mm[e]) - 1) & (kPhaseMax - 1)];
+#pragma unroll
+            for (int e = 0; e < kItems; e++) {
+                cplx<T> b = a[e];
+                one_update<T>(QSLHIP_PHASE, b, f[e].re, f[e].im);
+                const bool act = mm[e] != 0;
+                a[e].re = act ? b.re : a[e].re;
+                a[e].im = act ? b.im : a[e].im;
+                mm[e] &= mm[e] - 1;
+            }
+        }
+        j = end;
     }
 #pragma unroll
     for (int e = 0; e < kItems; e++)
@@ -964,12 +1091,22 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
     // k_np_diag pass each, their ops staged in one copy), groups of two to
     // kBondMax swap-like gates on disjoint qubit pairs (one k_np_bonds pass
     // each), and single gates.
-    enum { SINGLE, DIAG, BONDS, LOWBONDS, HIBONDS };
+    enum { SINGLE, DIAG, PHASES, BONDS, LOWBONDS, HIBONDS };
     struct Part { int type; size_t first, end; };
     std::vector<Part> parts;
     for (size_t i = 0; i < gates.size();) {
         size_t j = i;
-        while (j < gates.size() && diag_class(gates[j].kind) >= 0) j++;
+        bool phases = tuning().np_phase_lut != 0;
+        while (j < gates.size() && diag_class(gates[j].kind) >= 0) {
+            phases = phases && gates[j].kind == QSLHIP_PHASE;
+            j++;
+        }
+        if (j - i >= 2 && phases) {  // phase gates only: k_np_phases, kPhaseMax a pass
+            for (size_t f = i; f < j; f += kPhaseMax)
+                parts.push_back({PHASES, f, f + kPhaseMax < j ? f + kPhaseMax : j});
+            i = j;
+            continue;
+        }
         if (j - i >= 2) {
             parts.push_back({DIAG, i, j});
             i = j;
@@ -1044,24 +1181,72 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
             dops.push_back(o);
         }
     }
+    // the phase runs' tables, after the diagonal ops in the same staging
+    // copy: per run its lut (nbytes x 256 words) and its (c, s) pairs
+    const unsigned L = h->npt->L;
+    const int nbytes = (int)((L + 7) / 8);
+    QH_REQUIRE(nbytes <= kPhaseBytes, QSLHIP_EINVAL, "NP low part wider than the phase table");
+    const size_t dbytes = (dops.size() * sizeof(NPDiagOp) + 15) & ~(size_t)15;
+    std::vector<unsigned char> ptab;
+    for (const Part &pt : parts) {
+        if (pt.type != PHASES) continue;
+        std::vector<uint32_t> lut((size_t)nbytes * 256, 0u);
+        std::vector<double> cs;
+        for (size_t k = pt.first; k < pt.end; k++) {
+            const unsigned q = gates[k].q0, bit = (unsigned)(k - pt.first);
+            if (q < L)
+                for (unsigned v = 0; v < 256; v++)
+                    if ((v >> (q & 7)) & 1u) lut[(q >> 3) * 256 + v] |= 1u << bit;
+            cs.push_back(gates[k].c);
+            cs.push_back(gates[k].s);
+        }
+        const size_t at = ptab.size();
+        ptab.resize(at + lut.size() * 4 + cs.size() * 8);
+        std::memcpy(ptab.data() + at, lut.data(), lut.size() * 4);
+        std::memcpy(ptab.data() + at + lut.size() * 4, cs.data(), cs.size() * 8);
+    }
     const NPDiagOp *dev = nullptr;
-    if (!dops.empty()) {
-        const size_t bytes = dops.size() * sizeof(NPDiagOp);
+    const unsigned char *dev_ptab = nullptr;
+    if (!dops.empty() || !ptab.empty()) {
+        const size_t bytes = dbytes + ptab.size();
         if (h->staged) QH_CHECK(hipEventSynchronize(h->staged));  // pin may still be read
         h->pin.ensure(bytes);
         h->optab.ensure(bytes);
-        std::memcpy(h->pin.ptr, dops.data(), bytes);
+        std::memcpy(h->pin.ptr, dops.data(), dops.size() * sizeof(NPDiagOp));
+        std::memcpy(static_cast<unsigned char *>(h->pin.ptr) + dbytes, ptab.data(), ptab.size());
         QH_CHECK(hipMemcpyAsync(h->optab.ptr, h->pin.ptr, bytes, hipMemcpyHostToDevice,
                                 h->stream));
         QH_CHECK(hipEventRecord(h->staged, h->stream));
         dev = static_cast<const NPDiagOp *>(h->optab.ptr);
+        dev_ptab = static_cast<const unsigned char *>(h->optab.ptr) + dbytes;
     }
     NPDev d = dev_view(h);
     const dim3 grid(h->npt->nblk), blk(kBlock);
-    size_t first_op = 0;
+    size_t first_op = 0, ptab_at = 0;
     for (const Part &pt : parts) {
         if (pt.type == SINGLE) {
             np_launch_gate(h, gates[pt.first]);
+            continue;
+        }
+        if (pt.type == PHASES) {
+            NPPhaseRun run{};
+            run.nops = (int)(pt.end - pt.first);
+            run.nbytes = nbytes;
+            for (int k = 0; k < run.nops; k++) {
+                run.q[k] = gates[pt.first + k].q0;
+                if (run.q[k] < L) run.lowmask |= 1u << k;
+            }
+            const uint32_t *lut = reinterpret_cast<const uint32_t *>(dev_ptab + ptab_at);
+            const double *cs = reinterpret_cast<const double *>(dev_ptab + ptab_at +
+                                                                (size_t)nbytes * 256 * 4);
+            ptab_at += (size_t)nbytes * 256 * 4 + (size_t)run.nops * 16;
+            if (h->prec == 8)
+                hipLaunchKernelGGL(k_np_phases<double>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<double> *>(h->amps), d, lut, cs, run);
+            else
+                hipLaunchKernelGGL(k_np_phases<float>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<float> *>(h->amps), d, lut, cs, run);
+            QH_CHECK(hipGetLastError());
             continue;
         }
         if (pt.type == DIAG) {

@@ -214,6 +214,47 @@ def test_np_diagonal_runs_in_one_pass(dtype, n, k):
         assert _bits_equal(fused.getState(), want)
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,k", [(10, 4), (20, 10), (24, 12), (34, 3)])
+def test_np_phase_runs_in_one_pass(dtype, n, k):
+    # runs made only of phase gates take the masked pass (per string, the
+    # run indices acting on it; gates on segment bits applied to the whole
+    # workgroup): runs of 2 to 70 gates (split into passes of 32), qubits
+    # repeated, low and segment qubits interleaved, and the config-3 shape
+    # (every qubit once, in order).  n = 34 has a 3-byte low part.  The
+    # result is bit-identical to one kernel per gate and to the generic
+    # diagonal pass.
+    rng = np.random.default_rng(700 + n + k)
+    seq = []
+    for length in (2, 5, 31, 32, 33, 70):
+        for q0 in range(length % 2, n - 1, 2):  # bonds between the runs
+            seq.append((int(rng.choice([26, 27])), q0, q0 + 1, 0, _angle(rng, dtype)))
+        qs = np.arange(length) % n if length == 32 else rng.integers(n, size=length)
+        for q in qs:
+            seq.append((4, int(q), 0, 0, _angle(rng, dtype)))
+    ops = np.array(seq, OP_DTYPE)
+    total = math.comb(n, k)
+    amp = (rng.normal(size=total) + 1j * rng.normal(size=total))
+    amp = (amp / np.linalg.norm(amp)).astype(dtype)
+    lib = qsl_amd._lib
+    saved = lib.tuning_get("np_phase_lut")
+    got = {}
+    try:
+        for lut in (1, 0):
+            lib.tuning_set("np_phase_lut", lut)
+            q = qsl_amd.QubitsNP(n, k, dtype=dtype)
+            q.setAmps(amp)
+            q.apply_ops(ops)
+            got[lut] = q.getAmps()
+    finally:
+        lib.tuning_set("np_phase_lut", saved)
+    per_gate = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    per_gate.setAmps(amp)
+    per_gate.apply_ops(ops, mode=qsl_amd.MODE_PER_GATE)
+    assert _bits_equal(got[1], per_gate.getAmps())
+    assert _bits_equal(got[0], got[1])
+
+
 @pytest.fixture
 def bond_groups():
     saved = qsl_amd._lib.tuning_get("np_bonds")
