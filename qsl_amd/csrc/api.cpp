@@ -775,11 +775,12 @@ int qslhip_jit_source(unsigned nqubits, int prec, const qslhip_op *ops, uint64_t
         QH_REQUIRE(reg_tile_ok(p.logt, threads, prec), QSLHIP_EINVAL,
                    "this pass runs on LDS tiles, not register tiles");
         std::vector<RegOp> rops;
+        RegLayout lay;
         plan_reg_ops(tops.data() + p.op_first, p.op_count, p.logt, threads == 512 ? 9 : 8,
-                     prec == 8 ? 16 : 8, rops);
+                     prec == 8 ? 16 : 8, rops, &lay);
         const std::string src = jit_pass_source(nqubits, prec, p, threads,
                                                 mode != QSLHIP_MODE_REORDER, rops.data(),
-                                                (uint32_t)rops.size());
+                                                (uint32_t)rops.size(), lay);
         *len = src.size();
         if (buf && cap) {
             const size_t m = src.size() < cap - 1 ? src.size() : cap - 1;

@@ -235,6 +235,8 @@ struct Tuning {
                                  // one pass (0: one pass per bond)
     int np_hibonds = 1;          // NP: up to 4 disjoint bonds in the high part of the
                                  // strings fused into one pass (0: one pass per bond)
+    int free_layout = 1;         // register-tile passes load and store in the layouts their
+                                 // first and last gates use (0: registers on the top positions)
     int np_phase_lut = 1;        // NP: runs of phase gates through per-string masks of the
                                  // gates acting on it (0: every gate tested per amplitude)
     int np_bonds = 1;            // NP: swap-like gates on disjoint pairs per pass (1: one

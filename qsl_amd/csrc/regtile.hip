@@ -51,15 +51,26 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
     const uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1);
     const uint64_t outside = full & ~ta.tile_mask;
     const uint64_t tthread = deposit((uint64_t)t, ta.tlow);
-    uint64_t kpart[R];
+    const uint64_t tthread_out = deposit((uint64_t)t, ta.tlow_out);
+    uint64_t kpart[R], kpart_out[R];
+    uint32_t kt_out[R];
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        uint64_t kp = 0;
+        uint64_t kp = 0, ko = 0;
 #pragma unroll
         for (int i = 0; i < r; i++)
-            if ((k >> i) & 1) kp |= ta.kq[i];
+            if ((k >> i) & 1) {
+                kp |= ta.kq[i];
+                ko |= ta.kq_out[i];
+            }
         kpart[k] = kp;
+        kpart_out[k] = ko;
+        kt_out[k] = reg_tile_bits((uint32_t)k, ta.rho_out);
     }
+    // this thread's tile-index bits (zeros at register positions) in the
+    // load and the store layout
+    const uint32_t jt_in = thread_tile_bits(t, ta.rho_in);
+    const uint32_t jt_out = thread_tile_bits(t, ta.rho_out);
     uint64_t g = blockIdx.x;
     if (g >= ngroups) return;
     uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;
@@ -81,7 +92,7 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
         } else {
             wait_vmcnt<R>();  // only the previous group's stores may remain
         }
-        uint32_t jt = t;  // this thread's tile-index bits (zeros at register positions)
+        uint32_t jt = jt_in;
         uint32_t base = lds_swz<A>(jt) * A;
         uint32_t koff[R];
 #pragma unroll
@@ -189,8 +200,8 @@ __global__ __launch_bounds__(NT) void k_rtile(cplx<T> *__restrict__ st, unsigned
             }
             }
         }
-        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);
-        reg_store<T, R>(v, st, gbase | tthread, kpart, ta);
+        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, jt_out, kt_out);
+        reg_store<T, R>(v, st, gbase | tthread_out, kpart_out, ta);
         if (!more) break;
         g = gn;
     }
@@ -226,18 +237,51 @@ uint32_t swz_bytes(uint32_t e, int A)
 
 
 // Host planner: decode the TileOps of one pass (tile width K, 2^tb
-// threads, A bytes per amplitude) into register ops.  The pass starts and
-// ends in the load/store layout (registers on the top r positions); a remap
-// goes before a gate whose targets are not register bits, to the gate's
-// targets plus the next gates' targets in order (the set that postpones the
-// following remap longest), topped up from the current layout.
+// threads, A bytes per amplitude) into register ops.  A remap goes before a
+// gate whose targets are not register bits, to the gate's targets plus the
+// next gates' targets in order (the set that postpones the following remap
+// longest), topped up from the current layout.  Without `lay` the pass
+// loads and stores with registers on the top r positions.  With it (and
+// tuning free_layout), the load layout is the one the first gate would
+// remap to and the store layout the one the last gate leaves, each as long
+// as it keeps the tile's four lowest positions on lanes (the coalesced
+// runs, and the sampling runs' lanes): up to two remaps fewer.
 void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
-                  std::vector<RegOp> &out)
+                  std::vector<RegOp> &out, RegLayout *lay)
 {
     const int r = K - tb, R = 1 << r;
     const uint32_t all_k = (R >= 32) ? ~0u : ((1u << R) - 1);
     const uint32_t rho0 = ((1u << K) - 1) & ~((1u << tb) - 1);
-    uint32_t cur = rho0;
+    const uint32_t lanes = 0xFu;  // positions that stay on lanes at load and store
+    const bool free_layout = lay != nullptr && tuning().free_layout != 0;
+    // the layout a remap before op i picks when the current one is cur
+    auto choose = [&](uint32_t i, uint32_t cur) {
+        int tg[2], nt;
+        targets_of(ops[i], tg, nt);
+        uint32_t rho = 0;
+        for (int k = 0; k < nt; k++) rho |= 1u << tg[k];
+        for (uint32_t j = i + 1; j < count && __builtin_popcount(rho) < r; j++) {
+            int tj[2], nj;
+            targets_of(ops[j], tj, nj);
+            for (int k = 0; k < nj && __builtin_popcount(rho) < r; k++) rho |= 1u << tj[k];
+        }
+        for (int p = K - 1; p >= 0 && __builtin_popcount(rho) < r; p--)
+            if ((cur >> p) & 1u) rho |= 1u << p;
+        for (int p = K - 1; p >= 0 && __builtin_popcount(rho) < r; p--) rho |= 1u << p;
+        return rho;
+    };
+    uint32_t rho_in = rho0;
+    if (free_layout && count > 0) {
+        int tg[2], nt;
+        targets_of(ops[0], tg, nt);
+        uint32_t need = 0;
+        for (int k = 0; k < nt; k++) need |= 1u << tg[k];
+        if (need & ~rho0) {
+            const uint32_t rho = choose(0, rho0);
+            if (!(rho & lanes)) rho_in = rho;
+        }
+    }
+    uint32_t cur = rho_in;
     auto slot = [&](int pos) { return __builtin_popcount(cur & ((1u << pos) - 1)); };
     auto kset = [&](int pos) {  // registers whose bit at tile position pos is 1
         const int sl = slot(pos);
@@ -276,18 +320,7 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
         targets_of(t, tg, nt);
         uint32_t need = 0;
         for (int k = 0; k < nt; k++) need |= 1u << tg[k];
-        if ((need & ~cur) != 0) {
-            uint32_t rho = need;
-            for (uint32_t j = i + 1; j < count && __builtin_popcount(rho) < r; j++) {
-                int tj[2], nj;
-                targets_of(ops[j], tj, nj);
-                for (int k = 0; k < nj && __builtin_popcount(rho) < r; k++) rho |= 1u << tj[k];
-            }
-            for (int p = K - 1; p >= 0 && __builtin_popcount(rho) < r; p--)
-                if ((cur >> p) & 1u) rho |= 1u << p;
-            for (int p = K - 1; p >= 0 && __builtin_popcount(rho) < r; p--) rho |= 1u << p;
-            remap(rho);
-        }
+        if ((need & ~cur) != 0) remap(choose(i, cur));
         RegOp op{};
         RegOpHead &o = op.h;
         o.c = t.c;
@@ -347,13 +380,25 @@ void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
                    "pair op with both a thread and a register predicate");
         out.push_back(op);
     }
-    if (cur != rho0) remap(rho0);
+    uint32_t rho_out = rho0;
+    if (free_layout && !(cur & lanes)) rho_out = cur;
+    if (cur != rho_out) remap(rho_out);
+    if (lay) {
+        lay->in = rho_in;
+        lay->out = rho_out;
+    }
 }
 
-RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
+RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A, const RegLayout &lay)
 {
     const int K = p.logt, R = 1 << (K - tb);
+    const uint32_t rho0 = ((1u << K) - 1) & ~((1u << tb) - 1);
     RegTileArgs ta{};
+    ta.rho_in = lay.in ? lay.in : rho0;
+    ta.rho_out = lay.out ? lay.out : rho0;
+    QH_REQUIRE(__builtin_popcount(ta.rho_in) == K - tb && __builtin_popcount(ta.rho_out) == K - tb &&
+                   (ta.rho_in >> K) == 0 && (ta.rho_out >> K) == 0,
+               QSLHIP_EINVAL, "register layout does not fit the tile");
     ta.basis = ~0ull;
     ta.runs = nullptr;
     ta.tile_mask = p.tile_mask;
@@ -362,16 +407,18 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
     ta.zmask = 0;
     ta.zval = 0;
     ta.stream = n >= (unsigned)tuning().stream_min_qubits ? (uint32_t)tuning().pass_stream : 0u;
-    int pos = 0;
+    int pos = 0, kin = 0, kout = 0;
     for (unsigned q = 0; q < 64; q++) {
         if (!((p.tile_mask >> q) & 1ull)) continue;
-        if (pos < tb) ta.tlow |= 1ull << q;
-        else ta.kq[pos - tb] = 1ull << q;
+        if (pos < 32 && ((ta.rho_in >> pos) & 1u)) ta.kq[kin++ & 3] = 1ull << q;
+        else ta.tlow |= 1ull << q;
+        if (pos < 32 && ((ta.rho_out >> pos) & 1u)) ta.kq_out[kout++ & 3] = 1ull << q;
+        else ta.tlow_out |= 1ull << q;
         pos++;
     }
     QH_REQUIRE(pos == K, QSLHIP_EINVAL, "tile mask does not hold the tile width");
     QH_REQUIRE(K - tb >= 2 && K - tb <= 4, QSLHIP_EINVAL, "2..4 register bits per thread");
-    for (int k = 0; k < R; k++) ta.koff0[k] = swz_bytes((uint32_t)k << tb, A);
+    for (int k = 0; k < R; k++) ta.koff0[k] = swz_bytes(reg_tile_bits((uint32_t)k, ta.rho_in), A);
     QH_REQUIRE(((size_t)A << K) <= 64 * 1024, QSLHIP_EINVAL,
                "register tile needs more than 64 KiB of LDS");
     QH_REQUIRE(n >= (unsigned)K, QSLHIP_EINVAL, "tile wider than the state");
@@ -381,7 +428,7 @@ RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A)
 RegTileArgs pass_args(unsigned n, const TilePass &p, int tb, int A, const PassExtras &x,
                       uint64_t *groups)
 {
-    RegTileArgs ta = make_tile_args(n, p, tb, A);
+    RegTileArgs ta = make_tile_args(n, p, tb, A, x.lay);
     ta.basis = x.basis;
     ta.runs = x.runs;
     QH_REQUIRE((x.zmask & ~p.tile_mask) == 0 && (x.zval & ~x.zmask) == 0, QSLHIP_EINVAL,
@@ -515,12 +562,13 @@ bool launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
                "run sums need qubits 0..2 in the last tile");
     std::vector<RegOp> rops;
     std::vector<uint32_t> first(passes.size() + 1);
+    std::vector<RegLayout> lays(passes.size());
     for (size_t i = 0; i < passes.size(); i++) {
         QH_REQUIRE(reg_tile_ok(passes[i].logt, threads, h->prec), QSLHIP_EINVAL,
                    "no register-tile kernel for this tile width");
         first[i] = (uint32_t)rops.size();
         plan_reg_ops(ops.data() + passes[i].op_first, passes[i].op_count, passes[i].logt, tb, A,
-                     rops);
+                     rops, &lays[i]);
     }
     first[passes.size()] = (uint32_t)rops.size();
     // coefficient tables of the compiled passes: (c, s) per gate op
@@ -545,7 +593,8 @@ bool launch_reg_passes(qslhip_state *h, const std::vector<TilePass> &passes,
     const RegOp *dops = static_cast<const RegOp *>(h->optab.ptr);
     const double *dcoef =
         reinterpret_cast<const double *>(static_cast<char *>(h->optab.ptr) + bytes);
-    const std::vector<PassExtras> xs = pass_extras(h->n, passes, exact, basis, runs);
+    std::vector<PassExtras> xs = pass_extras(h->n, passes, exact, basis, runs);
+    for (size_t i = 0; i < passes.size(); i++) xs[i].lay = lays[i];
     for (size_t i = 0; i < passes.size(); i++) {
         const uint32_t n_ops = first[i + 1] - first[i];
         const TilePass &p = passes[i];

@@ -37,11 +37,18 @@ struct RegOp {
     uint32_t koff[kRegMaxR];  // RC_REMAP: swizzled LDS byte offset of register k
 };
 
+// Register positions of a pass's load and store layouts (0: the top r
+// tile positions).
+struct RegLayout {
+    uint32_t in = 0, out = 0;
+};
+
 void plan_reg_ops(const TileOp *ops, uint32_t count, int K, int tb, int A,
-                  std::vector<RegOp> &out);
+                  std::vector<RegOp> &out, RegLayout *lay = nullptr);
 uint32_t swz_bytes(uint32_t e, int A);
-// where one pass's tile lives (load layout, thread and register parts)
-RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A);
+// where one pass's tile lives (load and store layouts, thread and register parts)
+RegTileArgs make_tile_args(unsigned n, const TilePass &p, int tb, int A,
+                           const RegLayout &lay = RegLayout());
 // grid of a pass: gpw groups per workgroup when groups are plentiful
 uint64_t reg_grid(uint64_t groups, int gpw);
 
@@ -54,6 +61,7 @@ struct PassExtras {
     uint64_t gfix = 0;
     uint64_t zmask = 0;      // RegTileArgs::zmask / zval
     uint64_t zval = 0;
+    RegLayout lay;           // the pass's load and store layouts (plan_reg_ops)
 };
 // the tile arguments of a pass with its extras applied, and its group count
 RegTileArgs pass_args(unsigned n, const TilePass &p, int tb, int A, const PassExtras &x,
@@ -72,7 +80,7 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
 void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef);
 // introspection: the generated source of a pass, and a compile check
 std::string jit_pass_source(unsigned n, int prec, const TilePass &p, int threads, bool exact,
-                            const RegOp *ops, uint32_t nops);
+                            const RegOp *ops, uint32_t nops, const RegLayout &lay);
 bool jit_check_source(const std::string &src, const std::string &arch, std::string &log);
 
 }  // namespace qslhip

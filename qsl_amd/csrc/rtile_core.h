@@ -32,8 +32,11 @@ __host__ __device__ constexpr uint32_t lds_swz(uint32_t e)
 // the load/store layout maps to the state index gbase | dep(t) | kpart(k).
 struct RegTileArgs {
     uint64_t tile_mask;          // the K tile qubits
-    uint64_t tlow;               // the tb lowest of them (thread bits)
-    uint64_t kq[4];              // qubit of tile position tb + i, as a one-bit mask
+    uint64_t tlow;               // load layout: the qubits on thread bits (ascending)
+    uint64_t kq[4];              // load layout: qubit of register bit i, as a one-bit mask
+    uint64_t tlow_out;           // the same for the store layout
+    uint64_t kq_out[4];
+    uint32_t rho_in, rho_out;    // tile positions on register bits: load, store layout
     uint32_t koff0[kRegMaxR];    // swizzled LDS byte offsets of the load layout
     uint64_t basis;              // ~0: load the state; else the state is this
                                  // basis state, synthesized without a read
@@ -49,22 +52,43 @@ struct RegTileArgs {
 };
 
 // The sampling sums of the amplitudes about to be stored, in the pass's
-// group order: with qubits 0..2 on lane bits 0..2, eight lanes of register
-// k hold one 8-amplitude run, tile index (t | k << tb) with bits 0..2
-// clear, summed as the sampler sums a run of the stored state (amp_prob,
-// lane_sum_up<3>).  Run l of group g goes to runs[g * 2^(K-3) + l]; the
-// eight runs of a wavefront's register are 64 contiguous bytes.
+// group order: with tile positions 0..2 (qubits 0..2) on lane bits 0..2,
+// eight lanes of register k hold one 8-amplitude run, tile index
+// jt | kt[k] with bits 0..2 clear (jt: the thread's tile-index bits, kt[k]:
+// register k's, in the store layout), summed as the sampler sums a run of
+// the stored state (amp_prob, lane_sum_up<3>).  Run l of group g goes to
+// runs[g * 2^(K-3) + l].
 template <typename T, int R, int NT>
 __device__ __forceinline__ void reg_run_sums(const cplx<T> (&v)[R], double *runs, uint64_t g,
-                                             uint32_t t)
+                                             uint32_t jt, const uint32_t (&kt)[R])
 {
     constexpr int tb = NT == 256 ? 8 : 9;
     constexpr int lb = tb - 3 + (R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : 5);  // K - 3
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const double s = lane_sum_up<3>(amp_prob(v[k]));
-        if ((t & 7u) == 0) runs[(g << lb) + ((t >> 3) | ((uint32_t)k << (tb - 3)))] = s;
+        if ((jt & 7u) == 0) runs[(g << lb) + ((jt | kt[k]) >> 3)] = s;
     }
+}
+
+// Tile index bits of thread t in a layout with registers on positions rho:
+// t's bits in the other positions, ascending (zeros inserted at rho).
+__host__ __device__ inline uint32_t thread_tile_bits(uint32_t t, uint32_t rho)
+{
+    for (; rho; rho &= rho - 1) {
+        const uint32_t lowm = (rho & (0u - rho)) - 1;
+        t = (t & lowm) | ((t & ~lowm) << 1);
+    }
+    return t;
+}
+
+// Tile index bits of register k in a layout with registers on positions rho.
+__host__ __device__ inline uint32_t reg_tile_bits(uint32_t k, uint32_t rho)
+{
+    uint32_t out = 0;
+    for (int b = 0; rho; b++, rho &= rho - 1)
+        if ((k >> b) & 1u) out |= rho & (0u - rho);
+    return out;
 }
 
 // One thread's R amplitudes of a group: state indices idx0 | kpart[k].  With

@@ -64,8 +64,21 @@ std::string hex(uint64_t x)
 // The pass as straight-line device code.  Mirrors k_rtile (regtile.hip):
 // same prologue, group loop with prefetch, same helper calls per op.
 std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R, bool exact,
-                       const uint32_t *koff0)
+                       const RegTileArgs &ta)
 {
+    const uint32_t *koff0 = ta.koff0;
+    const int tb = NT == 256 ? 8 : 9;
+    const uint32_t rho0 = ((uint32_t)(R - 1)) << tb;  // registers on the top positions
+    // zeros inserted at the positions of rho, lowest first (thread_tile_bits)
+    auto insert_code = [](const char *var, uint32_t rho) {
+        std::string c;
+        for (; rho; rho &= rho - 1) {
+            const uint32_t lowm = (rho & (0u - rho)) - 1;
+            c += std::string("    ") + var + " = (" + var + " & " + std::to_string(lowm) +
+                 "u) | ((" + var + " & " + std::to_string(~lowm) + "u) << 1);\n";
+        }
+        return c;
+    };
     const int r = R == 4 ? 2 : R == 8 ? 3 : 4;
     std::ostringstream o;
     o << "#include \"rtile_core.h\"\nusing namespace qslhip;\n"
@@ -94,6 +107,28 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "            if ((k >> i) & 1) kp |= ta.kq[i];\n"
          "        kpart[k] = kp;\n"
          "    }\n"
+         ;
+    const bool same = ta.rho_in == ta.rho_out;
+    o << "    uint32_t jt0 = t;\n";
+    if (ta.rho_in != rho0) o << insert_code("jt0", ta.rho_in);
+    if (!same) {
+        o << "    const uint64_t tthread_out = deposit((uint64_t)t, ta.tlow_out);\n"
+             "    uint64_t kpart_out[R];\n"
+             "#pragma unroll\n"
+             "    for (int k = 0; k < R; k++) {\n"
+             "        uint64_t kp = 0;\n"
+             "#pragma unroll\n"
+             "        for (int i = 0; i < r; i++)\n"
+             "            if ((k >> i) & 1) kp |= ta.kq_out[i];\n"
+             "        kpart_out[k] = kp;\n"
+             "    }\n"
+             "    uint32_t jt1 = t;\n";
+        if (ta.rho_out != rho0) o << insert_code("jt1", ta.rho_out);
+    }
+    o << "    const uint32_t kto[R] = {";
+    for (int k = 0; k < R; k++)
+        o << (k ? ", " : "") << reg_tile_bits((uint32_t)k, ta.rho_out) << "u";
+    o << "};\n"
          "    uint64_t g = blockIdx.x;\n"
          "    if (g >= ngroups) return;\n"
          "    uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;\n"
@@ -113,7 +148,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "        } else {\n"
          "            wait_vmcnt<R>();\n"
          "        }\n"
-         "        uint32_t jt = t;\n"
+         "        uint32_t jt = jt0;\n"
          "        uint32_t base = lds_swz<A>(jt) * A;\n"
          "        (void)gbase; (void)jt; (void)base;\n"
          "        // a zero group of a basis state stays zero (reordered mode)\n"
@@ -185,10 +220,14 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
         o << "        }\n";
         j++;
     }
-    o << "        }\n"
-         "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, t);\n"
-         "        reg_store<T, R>(v, st, gbase | tthread, kpart, ta);\n"
-         "        if (!more) break;\n"
+    o << "        }\n";
+    if (same)
+        o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, jt0, kto);\n"
+             "        reg_store<T, R>(v, st, gbase | tthread, kpart, ta);\n";
+    else
+        o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, jt1, kto);\n"
+             "        reg_store<T, R>(v, st, gbase | tthread_out, kpart_out, ta);\n";
+    o << "        if (!more) break;\n"
          "        g = gn;\n"
          "    }\n}\n";
     return o.str();
@@ -319,11 +358,11 @@ Compiler &compiler()
 }  // namespace
 
 std::string jit_pass_source(unsigned n, int prec, const TilePass &p, int threads, bool exact,
-                            const RegOp *ops, uint32_t nops)
+                            const RegOp *ops, uint32_t nops, const RegLayout &lay)
 {
     const int tb = threads == 512 ? 9 : 8;
-    const RegTileArgs ta = make_tile_args(n, p, tb, prec == 8 ? 16 : 8);
-    return gen_source(ops, nops, prec, threads, 1 << (p.logt - tb), exact, ta.koff0);
+    const RegTileArgs ta = make_tile_args(n, p, tb, prec == 8 ? 16 : 8, lay);
+    return gen_source(ops, nops, prec, threads, 1 << (p.logt - tb), exact, ta);
 }
 
 bool jit_check_source(const std::string &src, const std::string &arch, std::string &log)
@@ -353,9 +392,9 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     if (mode == 0) return false;
     const int tb = threads == 512 ? 9 : 8;
     const int A = h->prec == 8 ? 16 : 8;
-    const RegTileArgs ta = make_tile_args(h->n, p, tb, A);
+    const RegTileArgs ta = make_tile_args(h->n, p, tb, A, x.lay);
     const int R = 1 << (p.logt - tb);
-    const std::string src = gen_source(ops, nops, h->prec, threads, R, exact, ta.koff0);
+    const std::string src = gen_source(ops, nops, h->prec, threads, R, exact, ta);
     hipFunction_t fn = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_mu);
