@@ -235,6 +235,9 @@ struct Tuning {
                                  // one pass (0: one pass per bond)
     int np_hibonds = 1;          // NP: up to 4 disjoint bonds in the high part of the
                                  // strings fused into one pass (0: one pass per bond)
+    int jit_prefetch = 2;        // compiled passes: a workgroup takes several groups and
+                                 // loads the next while computing one: 0 never (one group
+                                 // each), 1 always, 2 for passes that read part of a tile
     int free_layout = 1;         // register-tile passes load and store in the layouts their
                                  // first and last gates use (0: registers on the top positions)
     int np_phase_lut = 1;        // NP: runs of phase gates through per-string masks of the

@@ -64,7 +64,7 @@ std::string hex(uint64_t x)
 // The pass as straight-line device code.  Mirrors k_rtile (regtile.hip):
 // same prologue, group loop with prefetch, same helper calls per op.
 std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R, bool exact,
-                       const RegTileArgs &ta)
+                       const RegTileArgs &ta, bool prefetch)
 {
     const uint32_t *koff0 = ta.koff0;
     const int tb = NT == 256 ? 8 : 9;
@@ -130,25 +130,31 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
         o << (k ? ", " : "") << reg_tile_bits((uint32_t)k, ta.rho_out) << "u";
     o << "};\n"
          "    uint64_t g = blockIdx.x;\n"
-         "    if (g >= ngroups) return;\n"
-         "    uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;\n"
-         "    cplx<T> v[R], vn[R];\n"
-         "    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
-         "    wait_vmcnt<0>();\n"
-         "    for (;;) {\n"
-         "        const uint64_t gbase = gbase_n;\n"
-         "#pragma unroll\n"
-         "        for (int k = 0; k < R; k++) v[k] = vn[k];\n"
-         "        const uint64_t gn = g + gridDim.x;\n"
-         "        const bool more = gn < ngroups;\n"
-         "        if (more) {\n"
-         "            gbase_n = deposit(gn, ta.gvar) | ta.gfix;\n"
-         "            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
-         "            wait_vmcnt<2 * R>();\n"
-         "        } else {\n"
-         "            wait_vmcnt<R>();\n"
-         "        }\n"
-         "        uint32_t jt = jt0;\n"
+         "    if (g >= ngroups) return;\n";
+    if (prefetch)
+        o << "    uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;\n"
+             "    cplx<T> v[R], vn[R];\n"
+             "    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
+             "    wait_vmcnt<0>();\n"
+             "    for (;;) {\n"
+             "        const uint64_t gbase = gbase_n;\n"
+             "#pragma unroll\n"
+             "        for (int k = 0; k < R; k++) v[k] = vn[k];\n"
+             "        const uint64_t gn = g + gridDim.x;\n"
+             "        const bool more = gn < ngroups;\n"
+             "        if (more) {\n"
+             "            gbase_n = deposit(gn, ta.gvar) | ta.gfix;\n"
+             "            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
+             "            wait_vmcnt<2 * R>();\n"
+             "        } else {\n"
+             "            wait_vmcnt<R>();\n"
+             "        }\n";
+    else  // one group per workgroup: no second register set
+        o << "    {\n"
+             "        const uint64_t gbase = deposit(g, ta.gvar) | ta.gfix;\n"
+             "        cplx<T> v[R];\n"
+             "        reg_load<T, R>(v, st, gbase | tthread, kpart, ta);\n";
+    o << "        uint32_t jt = jt0;\n"
          "        uint32_t base = lds_swz<A>(jt) * A;\n"
          "        (void)gbase; (void)jt; (void)base;\n"
          "        // a zero group of a basis state stays zero (reordered mode)\n"
@@ -227,9 +233,10 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
     else
         o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, jt1, kto);\n"
              "        reg_store<T, R>(v, st, gbase | tthread_out, kpart_out, ta);\n";
-    o << "        if (!more) break;\n"
-         "        g = gn;\n"
-         "    }\n}\n";
+    if (prefetch)
+        o << "        if (!more) break;\n"
+             "        g = gn;\n";
+    o << "    }\n}\n";
     return o.str();
 }
 
@@ -362,7 +369,8 @@ std::string jit_pass_source(unsigned n, int prec, const TilePass &p, int threads
 {
     const int tb = threads == 512 ? 9 : 8;
     const RegTileArgs ta = make_tile_args(n, p, tb, prec == 8 ? 16 : 8, lay);
-    return gen_source(ops, nops, prec, threads, 1 << (p.logt - tb), exact, ta);
+    return gen_source(ops, nops, prec, threads, 1 << (p.logt - tb), exact, ta,
+                      tuning().jit_prefetch != 0);
 }
 
 bool jit_check_source(const std::string &src, const std::string &arch, std::string &log)
@@ -394,7 +402,12 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     const int A = h->prec == 8 ? 16 : 8;
     const RegTileArgs ta = make_tile_args(h->n, p, tb, A, x.lay);
     const int R = 1 << (p.logt - tb);
-    const std::string src = gen_source(ops, nops, h->prec, threads, R, exact, ta);
+    // Dense passes run one group per workgroup (more workgroups resident,
+    // measured faster); passes that read only part of each tile (from a
+    // basis state) prefetch the next group while computing one.
+    const int pf = tuning().jit_prefetch;
+    const bool prefetch = pf == 1 || (pf == 2 && x.zmask != 0);
+    const std::string src = gen_source(ops, nops, h->prec, threads, R, exact, ta, prefetch);
     hipFunction_t fn = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -423,7 +436,11 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     }
     uint64_t groups = 0;
     RegTileArgs targ = pass_args(h->n, p, tb, A, x, &groups);
-    const unsigned grid = (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1);
+    // a pass compiled without the group loop takes one group per workgroup
+    const bool looped = src.find("gbase_n") != std::string::npos;
+    if (!looped && groups * (uint64_t)threads >= (1ull << 31)) return false;
+    const unsigned grid = looped ? (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1)
+                                 : (unsigned)groups;
     const size_t shmem = (size_t)A << p.logt;
     void *st = amps;
     unsigned n = h->n;

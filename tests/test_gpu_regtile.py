@@ -199,16 +199,36 @@ def jit_mode():
     L.tuning_set("jit", saved)
 
 
+@pytest.fixture
+def pass_shape():
+    # free_layout: load / store layouts of the first / last gate (else the
+    # top register positions); jit_prefetch: 0 one group per workgroup,
+    # 1 a group loop with the next group's loads in flight
+    keys = ("free_layout", "jit_prefetch")
+    saved = {k: L.tuning_get(k) for k in keys}
+
+    def setter(free_layout, prefetch):
+        L.tuning_set("free_layout", free_layout)
+        L.tuning_set("jit_prefetch", prefetch)
+
+    yield setter
+    for k, v in saved.items():
+        L.tuning_set(k, v)
+
+
+@pytest.mark.parametrize("shape", [(1, 0), (1, 1), (0, 1)], ids=["free_single", "free_loop",
+                                                               "top_loop"])
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 @pytest.mark.parametrize("cfg", [(256, 10, 3), (256, 11, 3), (512, 12, 4)],
                          ids=lambda c: "k%d_t%d_low%d" % c)
-def test_compiled_passes_match_oracle(tuning, jit_mode, dtype, cfg):
+def test_compiled_passes_match_oracle(tuning, jit_mode, pass_shape, dtype, cfg, shape):
     # every pass compiled at run time (jit = 2): program order bit for bit,
-    # the reordered mode to rounding
+    # the reordered mode to rounding, with each pass shape
     kern, logt, low = cfg
     prec = 8 if dtype == np.complex128 else 4
     tuning(kern, prec, logt, low)
     jit_mode(2)
+    pass_shape(*shape)
     before = L.tuning_get("jit_compiled")
     n = 18
     rng = np.random.default_rng(7000 + logt + prec)
