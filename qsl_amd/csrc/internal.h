@@ -237,7 +237,8 @@ struct Tuning {
                                  // strings fused into one pass (0: one pass per bond)
     int jit_prefetch = 2;        // compiled passes: a workgroup takes several groups and
                                  // loads the next while computing one: 0 never (one group
-                                 // each), 1 always, 2 for passes that read part of a tile
+                                 // each), 1 always, 2 for FP32 passes and passes that read
+                                 // part of a tile
     int free_layout = 1;         // register-tile passes load and store in the layouts their
                                  // first and last gates use (0: registers on the top positions)
     int np_phase_lut = 1;        // NP: runs of phase gates through per-string masks of the

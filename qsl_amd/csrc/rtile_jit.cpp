@@ -402,11 +402,17 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     const int A = h->prec == 8 ? 16 : 8;
     const RegTileArgs ta = make_tile_args(h->n, p, tb, A, x.lay);
     const int R = 1 << (p.logt - tb);
-    // Dense passes run one group per workgroup (more workgroups resident,
-    // measured faster); passes that read only part of each tile (from a
-    // basis state) prefetch the next group while computing one.
+    uint64_t groups = 0;
+    RegTileArgs targ = pass_args(h->n, p, tb, A, x, &groups);
+    // FP64 passes that read whole tiles run one group per workgroup: without
+    // the second register set more workgroups are resident (measured faster
+    // at 30 qubits).  Passes that read only part of each tile (from a basis
+    // state) and FP32 passes (few registers either way; measured faster at
+    // 34 qubits) prefetch the next group while computing one, and so do
+    // passes whose groups outnumber what one grid dimension addresses.
     const int pf = tuning().jit_prefetch;
-    const bool prefetch = pf == 1 || (pf == 2 && x.zmask != 0);
+    const bool prefetch = pf == 1 || (pf == 2 && (x.zmask != 0 || h->prec != 8)) ||
+                          groups * (uint64_t)threads > 0xFFFFFFFFull;
     const std::string src = gen_source(ops, nops, h->prec, threads, R, exact, ta, prefetch);
     hipFunction_t fn = nullptr;
     {
@@ -434,13 +440,9 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
         if (it->second.failed) return false;
         fn = it->second.fn;
     }
-    uint64_t groups = 0;
-    RegTileArgs targ = pass_args(h->n, p, tb, A, x, &groups);
     // a pass compiled without the group loop takes one group per workgroup
-    const bool looped = src.find("gbase_n") != std::string::npos;
-    if (!looped && groups * (uint64_t)threads >= (1ull << 31)) return false;
-    const unsigned grid = looped ? (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1)
-                                 : (unsigned)groups;
+    const unsigned grid = prefetch ? (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1)
+                                   : (unsigned)groups;
     const size_t shmem = (size_t)A << p.logt;
     void *st = amps;
     unsigned n = h->n;
