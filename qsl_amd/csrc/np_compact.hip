@@ -31,6 +31,12 @@ struct NPBlk {
     uint32_t i0;  // first index inside the segment
 };
 
+// A block of k_np_lowbonds: the ranks [first, end) of segment hi, whole
+// runs of strings that share their bits from bit 12 up.
+struct NPRunBlk {
+    uint32_t hi, first, end;
+};
+
 struct NPTables {
     unsigned L = 0, hib = 0;
     uint64_t *seg_off = nullptr;      // [2^hib + 1] rank of each segment's start
@@ -40,6 +46,8 @@ struct NPTables {
     uint64_t *binom = nullptr;        // [64 * 64] C(q, m) at q * 64 + m
     NPBlk *blk = nullptr;             // block map
     uint32_t nblk = 0;
+    NPRunBlk *runblk = nullptr;       // whole runs below bit 12 per block
+    uint32_t nrunblk = 0;             // (k_np_lowbonds; built on first use)
     std::vector<uint64_t> h_seg_off;  // host copies
     std::vector<uint32_t> h_lo_list, h_rank_lo, h_class_start;
 };
@@ -583,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_np_bonds(cplx<T> *__restrict__ amp, 
 // gate, so the result is bit-identical.
 constexpr int kLowBits = 12;
 constexpr int kLowBondMax = 8;
-constexpr int kRunMax = 924;  // C(12, 6)
+constexpr int kLowSpan = 1024;  // amplitudes per block: whole runs, at most C(12, 6) = 924 each
 struct NPLowBonds {
     int m;
     int body[kLowBondMax];
@@ -591,50 +599,35 @@ struct NPLowBonds {
     double c[kLowBondMax], s[kLowBondMax];
 };
 
+// Adjacent swap-like gates on disjoint pairs below kLowBits, in one pass.
+// Such a gate pairs strings that share every bit from kLowBits up, so the
+// pairs stay inside a run; each block stages whole runs (the host's
+// run-aligned block map) in LDS and applies the gates in program order,
+// one barrier apart, each owner (bit q0 set, q1 clear) updating itself and
+// its partner at rank i +- C(q, j - 1) (adj_partner) with pair_update: the
+// same updates in the same order as one launch per gate, bit-identical.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_np_lowbonds(cplx<T> *__restrict__ amp, NPDev d,
+                                                        const NPRunBlk *__restrict__ rb,
                                                         NPLowBonds B)
 {
-    constexpr int kSpan = kChunk + kRunMax;
-    // (the strings come from the table, L2-resident, not from LDS: 31.8 KiB
-    // per workgroup for FP64 leaves room for five per CU)
-    __shared__ cplx<T> sv[kSpan];
-    __shared__ uint32_t s_first, s_end;
-    const Seg g = seg_of(d);
-    const uint32_t *lo = d.lo_list + g.cstart;
-    const uint32_t cend = g.i0 + kChunk < g.size ? g.i0 + kChunk : g.size;
-    const uint32_t lim = g.i0 + kSpan < g.size ? g.i0 + kSpan : g.size;
-    if (threadIdx.x == 0) {
-        s_first = cend;  // first run start in [i0, cend)
-        s_end = lim;     // end of the last run that starts before cend
-    }
-    __syncthreads();
-    // run starts: index i starts a run unless the string before it shares
-    // its bits from kLowBits up
-    for (uint32_t i = g.i0 + threadIdx.x; i < lim; i += kBlock) {
-        const uint32_t key = lo[i] >> kLowBits;
-        const uint32_t before = i > 0 ? (lo[i - 1] >> kLowBits) : ~0u;
-        if (key != before) {
-            if (i < cend) atomicMin(&s_first, i);
-            else atomicMin(&s_end, i);
-        }
-    }
-    __syncthreads();
-    const uint32_t first = s_first, end = s_first < cend ? s_end : s_first;
-    // C(q, m) for q < kLowBits in LDS (the partner offsets), and each
-    // thread's strings in registers for every gate
-    __shared__ uint32_t sbin[kLowBits][kLowBits + 1];
+    __shared__ cplx<T> sv[kLowSpan];
+    __shared__ uint32_t sbin[kLowBits][kLowBits + 1];  // C(q, m) for q < kLowBits
+    const NPRunBlk b = rb[blockIdx.x];
+    const uint32_t w = d.k - __popc(b.hi);
+    const uint32_t *lo = d.lo_list + d.class_start[w];
+    const uint64_t off = d.seg_off[b.hi];
     for (int t = threadIdx.x; t < kLowBits * (kLowBits + 1); t += kBlock)
         sbin[t / (kLowBits + 1)][t % (kLowBits + 1)] =
             (uint32_t)d.binom[(t / (kLowBits + 1)) * 64 + t % (kLowBits + 1)];
-    constexpr int kPer = (kSpan + kBlock - 1) / kBlock;
-    uint32_t xs[kPer];
+    constexpr int kPer = kLowSpan / kBlock;
+    uint32_t xs[kPer];  // each thread's strings, for every gate
 #pragma unroll
     for (int e = 0; e < kPer; e++) {
-        const uint32_t i = first + threadIdx.x + e * kBlock;
+        const uint32_t i = b.first + threadIdx.x + e * kBlock;
         xs[e] = 0;
-        if (i < end) {
-            sv[i - g.i0] = amp[g.off + i];
+        if (i < b.end) {
+            sv[i - b.first] = amp[off + i];
             xs[e] = lo[i];
         }
     }
@@ -644,23 +637,23 @@ __global__ __launch_bounds__(kBlock) void k_np_lowbonds(cplx<T> *__restrict__ am
         const T c = (T)B.c[j], s = (T)B.s[j];
 #pragma unroll
         for (int e = 0; e < kPer; e++) {
-            const uint32_t i = first + threadIdx.x + e * kBlock;
+            const uint32_t i = b.first + threadIdx.x + e * kBlock;
             const uint32_t x = xs[e];
-            // owners: bit q0 set, q1 clear (strings past `end` are 0: never owners)
-            if (i >= end || !((x >> q0) & 1u) || ((x >> q1) & 1u)) continue;
+            // owners: bit q0 set, q1 clear (slots past `end` hold 0: never owners)
+            if (!((x >> q0) & 1u) || ((x >> q1) & 1u)) continue;
             const uint32_t m = sbin[q][__popc(x & ((1u << q) - 1))];
             const uint32_t p = q1 > q0 ? i + m : i - m;
-            cplx<T> a = sv[i - g.i0], b = sv[p - g.i0];
-            pair_update<T>(B.body[j], a, b, c, s);
-            sv[i - g.i0] = a;
-            sv[p - g.i0] = b;
+            cplx<T> a = sv[i - b.first], bb = sv[p - b.first];
+            pair_update<T>(B.body[j], a, bb, c, s);
+            sv[i - b.first] = a;
+            sv[p - b.first] = bb;
         }
         __syncthreads();
     }
 #pragma unroll
     for (int e = 0; e < kPer; e++) {
-        const uint32_t i = first + threadIdx.x + e * kBlock;
-        if (i < end) amp[g.off + i] = sv[i - g.i0];
+        const uint32_t i = b.first + threadIdx.x + e * kBlock;
+        if (i < b.end) amp[off + i] = sv[i - b.first];
     }
 }
 
@@ -1044,6 +1037,7 @@ void np_free(qslhip_state *h)
     if (t->rank_lo) (void)hipFree(t->rank_lo);
     if (t->class_start) (void)hipFree(t->class_start);
     if (t->blk) (void)hipFree(t->blk);
+    if (t->runblk) (void)hipFree(t->runblk);
     if (t->binom) (void)hipFree(t->binom);
     delete t;
     h->npt = nullptr;
@@ -1083,6 +1077,44 @@ static bool bond_kind(int kind)
 {
     return kind == QSLHIP_SWAP || kind == QSLHIP_NP_ROT_X || kind == QSLHIP_NP_ROT_Y ||
            kind == QSLHIP_NP_HADAMARD;
+}
+
+// The run-aligned block map of k_np_lowbonds, built once per handle: in
+// each segment, the strings sharing their low-part bits from bit 12 up form
+// runs of C(12, w') consecutive ranks (in increasing order of those bits),
+// packed whole into blocks of at most kLowSpan ranks.
+static void np_run_blocks(qslhip_state *h)
+{
+    NPTables *t = h->npt;
+    if (t->runblk) return;
+    const unsigned L = t->L, lb = L < (unsigned)kLowBits ? L : (unsigned)kLowBits, hb = L - lb;
+    std::vector<NPRunBlk> blks;
+    const uint64_t nseg = 1ull << t->hib;
+    for (uint64_t sg = 0; sg < nseg; sg++) {
+        const unsigned wh = (unsigned)__builtin_popcountll(sg);
+        if (wh > h->k || h->k - wh > L) continue;
+        const unsigned wl = h->k - wh;
+        NPRunBlk cur{(uint32_t)sg, 0, 0};
+        for (uint32_t v = 0; v < (1u << hb); v++) {
+            const unsigned pv = (unsigned)__builtin_popcount(v);
+            if (pv > wl || wl - pv > lb) continue;
+            const uint32_t len = (uint32_t)choose_host(lb, wl - pv);
+            if (cur.end - cur.first + len > (uint32_t)kLowSpan) {
+                blks.push_back(cur);
+                cur.first = cur.end;
+            }
+            cur.end += len;
+        }
+        QH_REQUIRE(cur.end == t->h_seg_off[sg + 1] - t->h_seg_off[sg], QSLHIP_EDEVICE,
+                   "NP runs do not add up to the segment");
+        if (cur.end > cur.first) blks.push_back(cur);
+    }
+    QH_REQUIRE(!blks.empty() && blks.size() < (1ull << 31), QSLHIP_EINVAL,
+               "NP run-aligned block map out of range");
+    QH_CHECK(hipMalloc(&t->runblk, blks.size() * sizeof(NPRunBlk)));
+    QH_CHECK(hipMemcpy(t->runblk, blks.data(), blks.size() * sizeof(NPRunBlk),
+                       hipMemcpyHostToDevice));
+    t->nrunblk = (uint32_t)blks.size();
 }
 
 void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
@@ -1269,12 +1301,14 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
                 B.c[j] = g.c;
                 B.s[j] = g.s;
             }
+            np_run_blocks(h);
+            const dim3 rgrid(h->npt->nrunblk);
             if (h->prec == 8)
-                hipLaunchKernelGGL(k_np_lowbonds<double>, grid, blk, 0, h->stream,
-                                   static_cast<cplx<double> *>(h->amps), d, B);
+                hipLaunchKernelGGL(k_np_lowbonds<double>, rgrid, blk, 0, h->stream,
+                                   static_cast<cplx<double> *>(h->amps), d, h->npt->runblk, B);
             else
-                hipLaunchKernelGGL(k_np_lowbonds<float>, grid, blk, 0, h->stream,
-                                   static_cast<cplx<float> *>(h->amps), d, B);
+                hipLaunchKernelGGL(k_np_lowbonds<float>, rgrid, blk, 0, h->stream,
+                                   static_cast<cplx<float> *>(h->amps), d, h->npt->runblk, B);
         } else {  // BONDS, HIBONDS
             NPBonds B{};
             B.m = (int)(pt.end - pt.first);
