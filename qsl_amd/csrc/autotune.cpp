@@ -124,6 +124,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "np_phase_lut") *value = t.np_phase_lut;
         else if (k == "free_layout") *value = t.free_layout;
         else if (k == "jit_prefetch") *value = t.jit_prefetch;
+        else if (k == "group_order") *value = t.group_order;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
         else if (k == "run_sums_used") *value = g_run_sums_used;
@@ -176,6 +177,9 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "np_hibonds") {
             QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "np_hibonds must be 0 or 1");
             t.np_hibonds = value;
+        } else if (k == "group_order") {
+            QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "group_order must be 0, 1 or 2");
+            t.group_order = value;
         } else if (k == "jit_prefetch") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit_prefetch must be 0, 1 or 2");
             t.jit_prefetch = value;

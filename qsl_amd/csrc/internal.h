@@ -235,6 +235,9 @@ struct Tuning {
                                  // one pass (0: one pass per bond)
     int np_hibonds = 1;          // NP: up to 4 disjoint bonds in the high part of the
                                  // strings fused into one pass (0: one pass per bond)
+    int group_order = 2;         // register-tile passes: 0 group g in workgroup slot g,
+                                 // 1 each XCD sweeping an eighth of the groups, 2 the
+                                 // latter for passes that read part of a tile
     int jit_prefetch = 2;        // compiled passes: a workgroup takes several groups and
                                  // loads the next while computing one: 0 never (one group
                                  // each), 1 always, 2 for FP32 passes and passes that read

@@ -62,6 +62,7 @@ struct PassExtras {
     uint64_t zmask = 0;      // RegTileArgs::zmask / zval
     uint64_t zval = 0;
     RegLayout lay;           // the pass's load and store layouts (plan_reg_ops)
+    uint32_t gorder = 0;     // RegTileArgs::gorder
 };
 // the tile arguments of a pass with its extras applied, and its group count
 RegTileArgs pass_args(unsigned n, const TilePass &p, int tb, int A, const PassExtras &x,

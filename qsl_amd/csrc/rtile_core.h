@@ -49,7 +49,18 @@ struct RegTileArgs {
                                  // from zval are zero and not in memory yet
     uint32_t stream;             // streaming (nontemporal) accesses, for states far
                                  // larger than the caches: bit 0 loads, bit 1 stores
+    uint32_t gorder;             // group order (group_of)
 };
+
+// The group that the v-th workgroup slot of a pass takes.  Order 0: group v.
+// Order 1: with workgroups dispatched to the eight XCDs in turn (slot v on
+// XCD v % 8), XCD x sweeps the groups x * ngroups / 8 .. (x + 1) * ngroups / 8
+// in order instead of every eighth group (ngroups and the grid multiples of
+// 8).
+__device__ __forceinline__ uint64_t group_of(uint64_t v, uint64_t ngroups, uint32_t gorder)
+{
+    return gorder ? (v & 7u) * (ngroups >> 3) + (v >> 3) : v;
+}
 
 // The sampling sums of the amplitudes about to be stored, in the pass's
 // group order: with tile positions 0..2 (qubits 0..2) on lane bits 0..2,

@@ -132,7 +132,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
          "    uint64_t g = blockIdx.x;\n"
          "    if (g >= ngroups) return;\n";
     if (prefetch)
-        o << "    uint64_t gbase_n = deposit(g, ta.gvar) | ta.gfix;\n"
+        o << "    uint64_t gbase_n = deposit(group_of(g, ngroups, ta.gorder), ta.gvar) | ta.gfix;\n"
              "    cplx<T> v[R], vn[R];\n"
              "    reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
              "    wait_vmcnt<0>();\n"
@@ -143,7 +143,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
              "        const uint64_t gn = g + gridDim.x;\n"
              "        const bool more = gn < ngroups;\n"
              "        if (more) {\n"
-             "            gbase_n = deposit(gn, ta.gvar) | ta.gfix;\n"
+             "            gbase_n = deposit(group_of(gn, ngroups, ta.gorder), ta.gvar) | ta.gfix;\n"
              "            reg_load<T, R>(vn, st, gbase_n | tthread, kpart, ta);\n"
              "            wait_vmcnt<2 * R>();\n"
              "        } else {\n"
@@ -151,7 +151,7 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
              "        }\n";
     else  // one group per workgroup: no second register set
         o << "    {\n"
-             "        const uint64_t gbase = deposit(g, ta.gvar) | ta.gfix;\n"
+             "        const uint64_t gbase = deposit(group_of(g, ngroups, ta.gorder), ta.gvar) | ta.gfix;\n"
              "        cplx<T> v[R];\n"
              "        reg_load<T, R>(v, st, gbase | tthread, kpart, ta);\n";
     o << "        uint32_t jt = jt0;\n"
@@ -228,10 +228,10 @@ std::string gen_source(const RegOp *ops, uint32_t nops, int prec, int NT, int R,
     }
     o << "        }\n";
     if (same)
-        o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, jt0, kto);\n"
+        o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, group_of(g, ngroups, ta.gorder), jt0, kto);\n"
              "        reg_store<T, R>(v, st, gbase | tthread, kpart, ta);\n";
     else
-        o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, g, jt1, kto);\n"
+        o << "        if (ta.runs) reg_run_sums<T, R, NT>(v, ta.runs, group_of(g, ngroups, ta.gorder), jt1, kto);\n"
              "        reg_store<T, R>(v, st, gbase | tthread_out, kpart_out, ta);\n";
     if (prefetch)
         o << "        if (!more) break;\n"

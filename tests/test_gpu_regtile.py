@@ -284,3 +284,47 @@ def test_compiled_pass_equals_interpreted_pass(tuning, jit_mode, mode):
         assert np.array_equal(q.getState().view(np.uint8), want.view(np.uint8))
     else:
         assert np.abs(q.getState() - want).max() <= 1e-12
+
+
+@pytest.fixture
+def group_order():
+    saved = {k: L.tuning_get(k) for k in ("group_order", "jit")}
+    yield lambda order, jit: (L.tuning_set("group_order", order), L.tuning_set("jit", jit))
+    for k, v in saved.items():
+        L.tuning_set(k, v)
+
+
+@pytest.mark.parametrize("jit", [0, 2])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_group_orders_give_the_same_bits(group_order, jit, dtype):
+    # passes of 2^13 groups (24 qubits, tiles of 2^11) in both group orders
+    # and the default mix (the XCD sweep for the passes from a reset): the
+    # same groups with the same arithmetic, so the states are bit-identical,
+    # program order and reordered, and so are the shots sampled from the
+    # last pass's run sums after a reset
+    n = 24
+    rng = np.random.default_rng(4100 + jit + (dtype == np.complex64))
+    st0 = random_state(rng, n, dtype)
+    ops = _ops(rng, n, 120, dtype)
+    u = np.sort(rng.random(4096))
+    got = {}
+    for order in (0, 1, 2):
+        group_order(order, jit)
+        q = qsl_amd.Qubits(n, dtype=dtype)
+        q.setState(st0)
+        q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+        fused = q.getState()
+        q.setState(st0)
+        q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+        reord = q.getState()
+        q.sampleAllOutcomes(np.array([0.5]))  # the handle holds its sampling buffer
+        q.reset()
+        q.apply_ops(ops, mode=qsl_amd.MODE_REORDER)
+        shots = q.sampleAllOutcomes(u)
+        got[order] = (fused, reord, shots, q.getState())
+    for order in (1, 2):
+        for a, b in zip(got[0], got[order]):
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)), order
+    want = st0.copy()
+    O.apply_ops(want, ops)
+    assert np.array_equal(got[0][0].view(np.uint8), want.view(np.uint8))
