@@ -123,6 +123,7 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "np_hibonds") *value = t.np_hibonds;
         else if (k == "np_phase_lut") *value = t.np_phase_lut;
         else if (k == "free_layout") *value = t.free_layout;
+        else if (k == "dense_reorder") *value = t.dense_reorder;
         else if (k == "jit_prefetch") *value = t.jit_prefetch;
         else if (k == "group_order") *value = t.group_order;
         else if (k == "jit_compiled") *value = g_jit_compiled;
@@ -183,6 +184,9 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "jit_prefetch") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit_prefetch must be 0, 1 or 2");
             t.jit_prefetch = value;
+        } else if (k == "dense_reorder") {
+            QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "dense_reorder must be 0 or 1");
+            t.dense_reorder = value;
         } else if (k == "free_layout") {
             QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "free_layout must be 0 or 1");
             t.free_layout = value;

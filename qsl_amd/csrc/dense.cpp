@@ -95,8 +95,19 @@ void plan_dense_passes(unsigned n, const std::vector<GateDesc> &gates, int F,
     const int K = (int)std::min<unsigned>(kDenseTile, n);
     std::vector<TilePass> tp;
     std::vector<TileOp> to;
-    plan_tile_passes(n, 4, gates, K, kDenseLowRun, tp, to);
-    size_t g0 = 0;
+    // The tile passes: in program order, or commutation-aware (tuning
+    // dense_reorder 1: gates on disjoint qubits, or both diagonal, taken in
+    // either order, the reordered mode's planner).  order maps a tile op to
+    // its gate.
+    std::vector<uint32_t> order;
+    if (tuning().dense_reorder) {
+        plan_tile_passes_reorder(n, gates, K, kDenseLowRun, tp, to, order);
+    } else {
+        plan_tile_passes(n, 4, gates, K, kDenseLowRun, tp, to);
+        order.resize(to.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
+    }
+    QH_REQUIRE(order.size() == to.size(), QSLHIP_EINVAL, "dense plan: op order mismatch");
     for (const TilePass &P : tp) {
         DensePass D;
         D.tile_mask = P.tile_mask;
@@ -165,7 +176,7 @@ void plan_dense_passes(unsigned n, const std::vector<GateDesc> &gates, int F,
             open.swap(keep);
         };
         for (uint32_t k = 0; k < P.op_count; k++) {
-            const size_t gi = g0 + k;
+            const size_t gi = order[P.op_first + k];
             const TileOp &t = to[P.op_first + k];
             const bool local = t.pa >= 0 && (!is_gate2(t.kind) || t.pb >= 0);
             std::vector<int> q;
@@ -219,7 +230,6 @@ void plan_dense_passes(unsigned n, const std::vector<GateDesc> &gates, int F,
         open.clear();
         D.op_count = (uint32_t)(ops.size() - D.op_first);
         passes.push_back(D);
-        g0 += P.op_count;
     }
 }
 

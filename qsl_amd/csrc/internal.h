@@ -242,6 +242,10 @@ struct Tuning {
                                  // loads the next while computing one: 0 never (one group
                                  // each), 1 always, 2 for FP32 passes and passes that read
                                  // part of a tile
+    int dense_reorder = 0;       // dense MFMA plans: 1 commutation-aware tile passes (34q
+                                 // config 5: 46 passes instead of 113 but 372 dense
+                                 // unitaries instead of 264, 90 instead of 75 TFLOP/s and
+                                 // 197 instead of 231 gates/s), 0 program order
     int free_layout = 1;         // register-tile passes load and store in the layouts their
                                  // first and last gates use (0: registers on the top positions)
     int np_phase_lut = 1;        // NP: runs of phase gates through per-string masks of the

@@ -48,9 +48,20 @@ def replay(st, ops, pi, oi, mats):
                 apply_dense(st, [int(q) for q in r[4:4 + nq]], M)
 
 
+@pytest.fixture
+def dense_reorder():
+    saved = L.tuning_get("dense_reorder")
+    yield lambda v: L.tuning_set("dense_reorder", v)
+    L.tuning_set("dense_reorder", saved)
+
+
+@pytest.mark.parametrize("reorder", [0, 1])
 @pytest.mark.parametrize("F", [4, 5])
 @pytest.mark.parametrize("n", [12, 13, 14])
-def test_dense_plan_replays_to_oracle(F, n):
+def test_dense_plan_replays_to_oracle(dense_reorder, reorder, F, n):
+    # program-order tile passes and commutation-aware ones (tuning
+    # dense_reorder): either plan replays to the gate-by-gate state
+    dense_reorder(reorder)
     rng = np.random.default_rng(10 * n + F)
     ops = np.zeros(250, OP_DTYPE)
     for i in range(len(ops)):

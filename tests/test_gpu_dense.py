@@ -46,8 +46,17 @@ def test_haar_layers_vs_fp64(n, fusion_width):
     assert abs(np.linalg.norm(got) - 1) < 1e-5
 
 
+@pytest.fixture(params=[0, 1], ids=["program_order", "reordered"])
+def dense_reorder(request):
+    old = L.tuning_get("dense_reorder")
+    L.tuning_set("dense_reorder", request.param)
+    yield request.param
+    L.tuning_set("dense_reorder", old)
+
+
 @pytest.mark.parametrize("n", [13, 15])
-def test_every_kind_vs_fp64(n, fusion_width):
+def test_every_kind_vs_fp64(n, fusion_width, dense_reorder):
+    # both dense plans: tile passes in program order and commutation-aware
     rng = np.random.default_rng(n)
     ops = np.zeros(300, OP_DTYPE)
     kinds = GATE1 + GATE2
