@@ -382,17 +382,16 @@ __global__ __launch_bounds__(kBlock) void k_np_diag(cplx<T> *__restrict__ amp, N
         if (in[e]) amp[g.off + g.i0 + e * kBlock + threadIdx.x] = a[e];
 }
 
-// A run of up to kPhaseMax phase gates (np/gates.cpp:33-81) in one pass,
-// taken in program order.  A gate on a qubit >= L tests a bit of the
-// segment, the same for every string of the workgroup: it is applied to all
-// of them or skipped as a whole.  A maximal stretch of gates on low qubits
-// changes each amplitude only by the gates whose qubit is 1 in its string:
-// each thread looks up, per string, the mask of those run indices (one table
-// word per byte of the low part: lut[b][v] holds the gates on qubit 8b + i
-// for each bit i set in v) and iterates over its set bits, lowest first.
-// Strings of one segment have the same number of low ones, so the lanes
-// iterate in lockstep.  Every acting gate is applied with one_update's
-// arithmetic in program order: bit-identical to one launch per gate.
+// A run of up to kPhaseMax phase gates (np/gates.cpp:33-81) in one pass.
+// A phase gate changes only the amplitudes whose string has its qubit set,
+// so each amplitude takes just the run indices acting on it: per string the
+// mask of them is one table word per byte of the low part (lut[b][v] holds
+// the gates on qubit 8b + i for each bit i set in v) OR'd with the gates on
+// qubits of the segment that are 1 in it (the same for the whole
+// workgroup), and its set bits are taken lowest first, i.e. in program
+// order, each with one_update's arithmetic: bit-identical to one launch per
+// gate.  For a run over distinct qubits the strings of one segment have
+// equal counts, so the lanes iterate in lockstep.
 constexpr int kPhaseMax = 32;
 constexpr int kPhaseBytes = 3;  // low parts of up to 24 bits
 struct NPPhaseRun {
@@ -440,62 +439,47 @@ __global__ __launch_bounds__(kBlock) void k_np_phases(cplx<T> *__restrict__ amp,
             if (b < run.nbytes) w |= s_lut[b * 256 + ((lo[e] >> (8 * b)) & 255u)];
         m[e] = w;
     }
-    const uint32_t nops = (uint32_t)run.nops;
-    for (uint32_t j = 0; j < nops;) {
-        if (!((run.lowmask >> j) & 1u)) {  // a segment bit: uniform
-            const uint32_t q = run.q[j] - d.L;
-            if ((g.hi >> q) & 1u) {
-                const T c = (T)cs[2 * j], s = (T)cs[2 * j + 1];
+    // the gates on qubits of the segment that are 1 in it act on every
+    // string of the workgroup
+    uint32_t mh = 0;
+    for (int j = 0; j < run.nops; j++)
+        if (!((run.lowmask >> j) & 1u) && ((g.hi >> (run.q[j] - d.L)) & 1u)) mh |= 1u << j;
 #pragma unroll
-                for (int e = 0; e < kItems; e++) one_update<T>(QSLHIP_PHASE, a[e], c, s);
-            }
-            j++;
-            continue;
-        }
-        // the stretch of low gates [j, end)
-        const uint32_t high_after = ~run.lowmask & (~0u << j);
-        const uint32_t end = high_after ? (uint32_t)(__ffs(high_after) - 1) : nops;
-        const uint32_t span = (end >= 32 ? ~0u : ((1u << end) - 1)) & (~0u << j);
-        uint32_t mm[kItems];
+    for (int e = 0; e < kItems; e++) m[e] = in[e] ? (m[e] | mh) : 0u;
+    // the usual case (a run over distinct qubits): every string of the
+    // wavefront has the same number of acting gates, and the loop needs no
+    // per-lane selects
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(__popc(m[0]));
+    bool same = true;
 #pragma unroll
-        for (int e = 0; e < kItems; e++) mm[e] = m[e] & span;
-        // the usual case (a stretch over distinct qubits): every string of
-        // the wavefront has the same number of acting gates, and the loop
-        // needs no per-lane selects
-        const uint32_t cnt = __builtin_amdgcn_readfirstlane(__popc(mm[0]));
-        bool same = true;
-#pragma unroll
-        for (int e = 0; e < kItems; e++) same = same && (uint32_t)__popc(mm[e]) == cnt;
-        if (__all(same)) {
-            for (uint32_t it = 0; it < cnt; it++) {
-                cplx<T> f[kItems];
-#pragma unroll
-                for (int e = 0; e < kItems; e++)  // cnt more set bits: mm[e] != 0
-                    f[e] = s_cs[__builtin_ctz(mm[e])];
-#pragma unroll
-                for (int e = 0; e < kItems; e++) {
-                    one_update<T>(QSLHIP_PHASE, a[e], f[e].re, f[e].im);
-                    mm[e] &= mm[e] - 1;
-                }
-            }
-            j = end;
-            continue;
-        }
-        while (mm[0] | mm[1] | mm[2] | mm[3]) {
+    for (int e = 0; e < kItems; e++) same = same && (uint32_t)__popc(m[e]) == cnt;
+    if (__all(same)) {
+        for (uint32_t it = 0; it < cnt; it++) {
             cplx<T> f[kItems];
 #pragma unroll
-            for (int e = 0; e < kItems; e++) f[e] = s_cs[(__ffs(mm[e]) - 1) & (kPhaseMax - 1)];
+            for (int e = 0; e < kItems; e++)  // cnt more set bits: m[e] != 0
+                f[e] = s_cs[__builtin_ctz(m[e])];
+#pragma unroll
+            for (int e = 0; e < kItems; e++) {
+                one_update<T>(QSLHIP_PHASE, a[e], f[e].re, f[e].im);
+                m[e] &= m[e] - 1;
+            }
+        }
+    } else {
+        while (m[0] | m[1] | m[2] | m[3]) {
+            cplx<T> f[kItems];
+#pragma unroll
+            for (int e = 0; e < kItems; e++) f[e] = s_cs[(__ffs(m[e]) - 1) & (kPhaseMax - 1)];
 #pragma unroll
             for (int e = 0; e < kItems; e++) {
                 cplx<T> b = a[e];
                 one_update<T>(QSLHIP_PHASE, b, f[e].re, f[e].im);
-                const bool act = mm[e] != 0;
+                const bool act = m[e] != 0;
                 a[e].re = act ? b.re : a[e].re;
                 a[e].im = act ? b.im : a[e].im;
-                mm[e] &= mm[e] - 1;
+                m[e] &= m[e] - 1;
             }
         }
-        j = end;
     }
 #pragma unroll
     for (int e = 0; e < kItems; e++)
