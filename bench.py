@@ -647,15 +647,19 @@ def main():
     # is still zero); the last pass also writes the 2^30 B of run sums.  The
     # plan for a state that is not a basis state spreads the diagonal gates
     # over the passes, so it is its own structure: the first application
-    # interprets, the second compiles.  The timed region is three compiled
-    # applications queued behind an untimed one, so the host's planning of
-    # each runs while the device works and the passes follow each other
-    # without gaps: steady_ms is one application's share
+    # interprets, the second compiles.  The next ones let each pass try its
+    # four launch shapes and keep the fastest (tuning pass_tune).  The timed
+    # region is three compiled applications queued behind an untimed one,
+    # so the host's planning of each runs while the device works and the
+    # passes follow each other without gaps: steady_ms is one application's
+    # share
     for _ in range(2):
         q.apply_ops(ops, mode=hl_mode)
     L.jit_wait()
     q.sync()
-    q.apply_ops(ops, mode=hl_mode)
+    for _ in range(5):
+        q.apply_ops(ops, mode=hl_mode)
+        q.sync()
     ev.record(e0, s)
     for _ in range(3):
         q.apply_ops(ops, mode=hl_mode)

@@ -125,6 +125,8 @@ int qslhip_tuning_get(const char *key, int *value)
         else if (k == "free_layout") *value = t.free_layout;
         else if (k == "dense_reorder") *value = t.dense_reorder;
         else if (k == "jit_prefetch") *value = t.jit_prefetch;
+        else if (k == "pass_tune") *value = t.pass_tune;
+        else if (k == "pass_tune_pending") *value = pass_tune_pending();
         else if (k == "group_order") *value = t.group_order;
         else if (k == "jit_compiled") *value = g_jit_compiled;
         else if (k == "jit_failed") *value = g_jit_failed;
@@ -181,6 +183,9 @@ int qslhip_tuning_set(const char *key, int value)
         } else if (k == "group_order") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "group_order must be 0, 1 or 2");
             t.group_order = value;
+        } else if (k == "pass_tune") {
+            QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "pass_tune must be 0 or 1");
+            t.pass_tune = value;
         } else if (k == "jit_prefetch") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "jit_prefetch must be 0, 1 or 2");
             t.jit_prefetch = value;

@@ -238,6 +238,9 @@ struct Tuning {
     int group_order = 2;         // register-tile passes: 0 group g in workgroup slot g,
                                  // 1 each XCD sweeping an eighth of the groups, 2 the
                                  // latter for passes that read part of a tile
+    int pass_tune = 1;           // compiled FP64 passes that read whole tiles: try the four
+                                 // launch shapes (one group or a loop, group order 0 or
+                                 // 1) on their first launches and keep the fastest
     int jit_prefetch = 2;        // compiled passes: a workgroup takes several groups and
                                  // loads the next while computing one: 0 never (one group
                                  // each), 1 always, 2 for FP32 passes and passes that read
@@ -275,6 +278,7 @@ Tuning &tuning();
 extern int g_jit_compiled, g_jit_failed;  // passes compiled at run time / failed
 void jit_wait_idle();  // blocks until the compiler thread has no queued pass
 extern int g_run_sums_used;  // samplings that took a gate pass's run sums
+int pass_tune_pending();       // passes still trying launch shapes (pass_tune)
 
 inline unsigned grid_for(uint64_t items, unsigned per_block)
 {

@@ -392,6 +392,57 @@ void jit_coefficients(const RegOp *ops, uint32_t nops, std::vector<double> &coef
     }
 }
 
+namespace {
+
+// The compiled function of a pass source, or nullptr (then the pass runs
+// interpreted): jit mode 1 queues a source for the compiler thread the
+// second time it is asked for, mode 2 compiles it now.  Caller holds g_mu.
+hipFunction_t pass_function(int dev, const std::string &src, int mode)
+{
+    const auto key = std::make_pair(dev, src);
+    auto it = g_cache.find(key);
+    if (it == g_cache.end()) {
+        if (g_cache.size() >= kMaxCached) return nullptr;
+        if (mode == 1) {
+            if (g_seen.size() >= kMaxSeen) g_seen.clear();
+            if (++g_seen[src] >= 2) compiler().enqueue(dev, src);
+            return nullptr;
+        }
+        Compiled c;
+        if (!compile(dev, src, c)) {
+            c.failed = true;
+            g_jit_failed++;
+        } else {
+            g_jit_compiled++;
+        }
+        it = g_cache.emplace(key, c).first;
+    }
+    return it->second.failed ? nullptr : it->second.fn;
+}
+
+// Per-pass choice among four launch shapes of an FP64 pass that reads whole
+// tiles (tuning pass_tune): one group per workgroup or the prefetching
+// group loop, each in group order 0 or 1.  Which is fastest depends on the
+// tile's qubits (a gate-free copy with the same addressing varies by up to
+// 20% between them), so the first launches of a pass try each shape once,
+// timed with events on the stream, and later launches take the fastest.
+// The shapes move the same groups through the same arithmetic: results are
+// bit-identical whichever runs.
+struct PassTuner {
+    float best[4] = {0, 0, 0, 0};
+    bool tried[4] = {false, false, false, false};
+    int pending = -1;  // shape whose events are in flight
+    int choice = -1;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+std::map<std::string, PassTuner> g_tuners;  // pass source + tile addressing -> tuner
+constexpr size_t kMaxTuners = 256;
+int g_tune_pending = 0;  // tuners still trying shapes
+
+}  // namespace
+
+int pass_tune_pending() { return g_tune_pending; }
+
 bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads, int gpw,
                      bool exact, const RegOp *ops, uint32_t nops, const double *dev_coef,
                      const PassExtras &x)
@@ -404,6 +455,7 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     const int R = 1 << (p.logt - tb);
     uint64_t groups = 0;
     RegTileArgs targ = pass_args(h->n, p, tb, A, x, &groups);
+    const bool single_ok = groups * (uint64_t)threads <= 0xFFFFFFFFull;
     // FP64 passes that read whole tiles run one group per workgroup: without
     // the second register set more workgroups are resident (measured faster
     // at 30 qubits).  Passes that read only part of each tile (from a basis
@@ -411,46 +463,95 @@ bool jit_launch_pass(qslhip_state *h, void *amps, const TilePass &p, int threads
     // 34 qubits) prefetch the next group while computing one, and so do
     // passes whose groups outnumber what one grid dimension addresses.
     const int pf = tuning().jit_prefetch;
-    const bool prefetch = pf == 1 || (pf == 2 && (x.zmask != 0 || h->prec != 8)) ||
-                          groups * (uint64_t)threads > 0xFFFFFFFFull;
-    const std::string src = gen_source(ops, nops, h->prec, threads, R, exact, ta, prefetch);
+    const bool prefetch = pf == 1 || (pf == 2 && (x.zmask != 0 || h->prec != 8)) || !single_ok;
+    const bool tune = tuning().pass_tune != 0 && h->prec == 8 && x.zmask == 0 && single_ok &&
+                      groups >= 64 && (groups % 8) == 0;
     hipFunction_t fn = nullptr;
+    bool loop = prefetch;
+    int shape = -1;
+    PassTuner *tu = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        auto key = std::make_pair(h->device, src);
-        auto it = g_cache.find(key);
-        if (it == g_cache.end()) {
-            if (g_cache.size() >= kMaxCached) return false;
-            if (mode == 1) {
-                // compile a structure once it repeats, on the compiler
-                // thread; this application runs interpreted
-                if (g_seen.size() >= kMaxSeen) g_seen.clear();
-                if (++g_seen[src] >= 2) compiler().enqueue(h->device, src);
-                return false;
+        if (!tune) {
+            fn = pass_function(h->device, gen_source(ops, nops, h->prec, threads, R, exact, ta,
+                                                     prefetch),
+                               mode);
+        } else {
+            const std::string src1 = gen_source(ops, nops, h->prec, threads, R, exact, ta, false);
+            const std::string src2 = gen_source(ops, nops, h->prec, threads, R, exact, ta, true);
+            hipFunction_t fs = pass_function(h->device, src1, mode);
+            hipFunction_t fl = pass_function(h->device, src2, mode);
+            if (!fs && !fl) return false;
+            std::string key = src1;
+            key += "|" + std::to_string(h->device) + "|" + std::to_string(p.tile_mask) + "|" +
+                   std::to_string(targ.gvar) + "|" + std::to_string(targ.gfix);
+            auto it = g_tuners.find(key);
+            if (it == g_tuners.end() && g_tuners.size() < kMaxTuners) {
+                it = g_tuners.emplace(key, PassTuner()).first;
+                g_tune_pending++;
             }
-            Compiled c;  // mode 2: compile now
-            if (!compile(h->device, src, c)) {
-                c.failed = true;
-                g_jit_failed++;
+            if (it != g_tuners.end()) {
+                tu = &it->second;
+                if (tu->pending >= 0 && hipEventQuery(tu->ev[1]) == hipSuccess) {
+                    float ms = 0;
+                    if (hipEventElapsedTime(&ms, tu->ev[0], tu->ev[1]) == hipSuccess) {
+                        tu->best[tu->pending] = ms;
+                        tu->tried[tu->pending] = true;
+                    }
+                    tu->pending = -1;
+                    if (tu->choice < 0 && tu->tried[0] && tu->tried[1] && tu->tried[2] &&
+                        tu->tried[3]) {
+                        int b = 0;
+                        for (int v = 1; v < 4; v++)
+                            if (tu->best[v] < tu->best[b]) b = v;
+                        tu->choice = b;
+                        g_tune_pending--;
+                    }
+                }
+                if (tu->choice >= 0) {
+                    shape = tu->choice;
+                } else if (tu->pending < 0) {
+                    for (int v = 0; v < 4 && shape < 0; v++)
+                        if (!tu->tried[v] && ((v & 2) ? fl : fs)) shape = v;
+                }
+                if (shape < 0 || !((shape & 2) ? fl : fs)) shape = -1;
+            }
+            if (shape >= 0) {
+                loop = (shape & 2) != 0;
+                targ.gorder = (uint32_t)(shape & 1);
+                fn = loop ? fl : fs;
+                if (tu->choice >= 0 || tu->pending >= 0) tu = nullptr;  // nothing to time
             } else {
-                g_jit_compiled++;
+                tu = nullptr;
+                fn = prefetch ? fl : fs;
+                if (!fn) {  // the static shape is not compiled yet: the other one
+                    fn = prefetch ? fs : fl;
+                    loop = !prefetch;
+                }
             }
-            it = g_cache.emplace(key, c).first;
         }
-        if (it->second.failed) return false;
-        fn = it->second.fn;
+        if (!fn) return false;
+        if (tu) {
+            if (!tu->ev[0]) {
+                QH_CHECK(hipEventCreate(&tu->ev[0]));
+                QH_CHECK(hipEventCreate(&tu->ev[1]));
+            }
+            tu->pending = shape;
+        }
     }
     // a pass compiled without the group loop takes one group per workgroup
-    const unsigned grid = prefetch ? (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1)
-                                   : (unsigned)groups;
+    const unsigned grid = loop ? (unsigned)reg_grid(groups, gpw > 0 ? gpw : 1) : (unsigned)groups;
+    if (grid % 8) targ.gorder = 0;
     const size_t shmem = (size_t)A << p.logt;
     void *st = amps;
     unsigned n = h->n;
     const double *cf = dev_coef;
     unsigned long long ng = groups;
     void *args[] = {&st, &n, &targ, &cf, &ng};
+    if (tu) QH_CHECK(hipEventRecord(tu->ev[0], h->stream));
     QH_CHECK(hipModuleLaunchKernel(fn, grid, 1, 1, (unsigned)threads, 1, 1, (unsigned)shmem,
                                    h->stream, args, nullptr));
+    if (tu) QH_CHECK(hipEventRecord(tu->ev[1], h->stream));
     return true;
 }
 

@@ -204,12 +204,14 @@ def pass_shape():
     # free_layout: load / store layouts of the first / last gate (else the
     # top register positions); jit_prefetch: 0 one group per workgroup,
     # 1 a group loop with the next group's loads in flight
-    keys = ("free_layout", "jit_prefetch")
+    # (pass_tune off: the shape asked for is the shape that runs)
+    keys = ("free_layout", "jit_prefetch", "pass_tune")
     saved = {k: L.tuning_get(k) for k in keys}
 
     def setter(free_layout, prefetch):
         L.tuning_set("free_layout", free_layout)
         L.tuning_set("jit_prefetch", prefetch)
+        L.tuning_set("pass_tune", 0)
 
     yield setter
     for k, v in saved.items():
@@ -328,3 +330,32 @@ def test_group_orders_give_the_same_bits(group_order, jit, dtype):
     want = st0.copy()
     O.apply_ops(want, ops)
     assert np.array_equal(got[0][0].view(np.uint8), want.view(np.uint8))
+
+
+def test_pass_shape_tuner_gives_the_same_bits(jit_mode):
+    # FP64 passes that read whole tiles try their four launch shapes on the
+    # first launches (one group per workgroup or a group loop, group order
+    # 0 or 1) and keep the fastest: every application along the way, and
+    # after the choice, gives the bits of the fixed shape
+    saved = {k: L.tuning_get(k) for k in ("pass_tune", "reg_min_qubits")}
+    try:
+        L.tuning_set("reg_min_qubits", 0)
+        jit_mode(2)
+        n = 22
+        rng = np.random.default_rng(5150)
+        st0 = random_state(rng, n, np.complex128)
+        ops = _ops(rng, n, 100, np.complex128)
+        L.tuning_set("pass_tune", 0)
+        q = qsl_amd.Qubits(n)
+        q.setState(st0)
+        q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+        want = q.getState()
+        L.tuning_set("pass_tune", 1)
+        for _ in range(6):
+            q.setState(st0)
+            q.apply_ops(ops, mode=qsl_amd.MODE_FUSED)
+            q.sync()
+            assert np.array_equal(q.getState().view(np.uint8), want.view(np.uint8))
+    finally:
+        for k, v in saved.items():
+            L.tuning_set(k, v)
