@@ -563,16 +563,9 @@ __global__ __launch_bounds__(kBlock) void k_np_bonds(cplx<T> *__restrict__ amp, 
     }
 }
 
-// Adjacent swap-like gates on pairwise disjoint bonds whose qubits are all
-// below kLowBits, in one pass.  Within a segment the strings are in numeric
-// order, so those sharing bits kLowBits and up form a contiguous run (at
-// most C(12, 6) = 924 ranks) that every such gate maps to itself: a bond's
-// partner is r +- C(q, ones below q) (adj_partner) inside the run.  A
-// workgroup takes the runs that start in its chunk, whole (reading up to
-// one run past the chunk's end), stages them in LDS, applies the gates in
-// program order with pair_update (one barrier per gate) and writes them
-// back: each pair sees the same updates in the same order as one launch per
-// gate, so the result is bit-identical.
+// Within a segment the strings are in numeric order, so those sharing bits
+// kLowBits and up form a contiguous run (at most C(12, 6) = 924 ranks) that
+// every adjacent swap-like gate below kLowBits maps to itself.
 constexpr int kLowBits = 12;
 constexpr int kLowBondMax = 8;
 constexpr int kLowSpan = 1024;  // amplitudes per block: whole runs, at most C(12, 6) = 924 each
