@@ -21,11 +21,13 @@
 // (bit-identical), or for QSLHIP_MODE_REORDER the tolerance-bound forms with
 // fused multiply-adds (gate_math.h pair_update_fast).
 //
-// Loads and stores use the layout with registers on the top r positions,
-// so threads sit on the lowest tile qubits (coalesced).  A workgroup may
-// take several groups and prefetch the next while computing one (k_rtile).  The per-amplitude
-// arithmetic is gate_math.h's in program order: results are bit-identical
-// to the one-gate kernels.
+// Loads and stores keep the tile's four lowest positions on lanes
+// (coalesced); the registers sit on the positions the pass's first and last
+// gates need, or on the top r positions (plan_reg_ops).  A workgroup may
+// take several groups and prefetch the next while computing one (k_rtile),
+// in either group order (group_of).  The per-amplitude arithmetic is
+// gate_math.h's in program order: results are bit-identical to the one-gate
+// kernels.
 #include <cstring>
 
 #include "rtile.h"
