@@ -369,8 +369,9 @@ def bench_c4(steps=3, plain_sample=64):
 
 def bench_c5(layers34=30, layers30=30):
     """Config 5: 34-qubit FP32 (128 GiB) Haar layers fused into dense
-    5-qubit unitaries on matrix cores; error of the same family at 30
-    qubits against the FP64 path (bit-identical to the oracle)."""
+    4- or 5-qubit unitaries (tuning dense_f) on matrix cores, the other
+    width timed beside it; error of the same family at 30 qubits against
+    the FP64 path (bit-identical to the oracle)."""
     import qsl_amd
     from qsl_amd import workloads as W
     from qsl_amd import _lib as L
@@ -405,6 +406,26 @@ def bench_c5(layers34=30, layers30=30):
                    "utilisation": round(tflops / MFMA_FP32_PEAK_TFLOPS, 3),
                    "note": "plan flops over the whole dense application (loads, stores and "
                            "single-gate tile ops included in the time)"}
+    # the other fusion width on the same circuit, for comparison
+    F2 = 9 - F
+    L.tuning_set("dense_f", F2)
+    try:
+        q.reset()
+        q.sync()
+        ev.record(e0, st)
+        q.apply_ops(ops, mode=qsl_amd.MODE_DENSE)
+        ev.record(e1, st)
+        ms2 = ev.elapsed_ms(e0, e1)
+        _, oi2, _ = L.plan_dense(n, ops, F2)
+        nd2 = int((oi2[:, 0] == 1).sum())
+        tf2 = nd2 * (8 << F2) * float(1 << n) / (ms2 * 1e-3) / 1e12
+        res["other_fusion_width"] = {
+            "fusion_width": F2, "ms": round(ms2, 1),
+            "gates_per_s": round(len(ops) / (ms2 * 1e-3), 1), "dense_unitaries": nd2,
+            "achieved_TFLOPs": round(tf2, 1),
+            "utilisation": round(tf2 / MFMA_FP32_PEAK_TFLOPS, 3)}
+    finally:
+        L.tuning_set("dense_f", F)
     # the first 6 layers on FP32 register-tile passes, reordered (every pass
     # is compiled at run time, so the comparison stays at 6 layers): the
     # first two applications interpret (the second queues the structures
@@ -487,6 +508,9 @@ def main():
     ap.add_argument("--profile-c5", action="store_true",
                     help="short fixed run of the dense MFMA path (34 qubits FP32, 3 Haar layers) "
                          "for rocprofv3")
+    ap.add_argument("--dense-f", type=int, choices=(4, 5), default=None,
+                    help="qubits per fused dense unitary for --profile-c5 (default: the "
+                         "library's tuning, 4)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline 30-qubit workload")
     args = ap.parse_args()
@@ -516,6 +540,8 @@ def main():
     if args.profile_c5:
         # the dense fused-unitary path on matrix cores: the first 3 of the
         # 30 Haar layers at 34 qubits, applied once from |0...0>
+        if args.dense_f is not None:
+            L.tuning_set("dense_f", args.dense_f)
         F = L.tuning_get("dense_f")
         ops5, _ = W.c5_haar_layers(34, 3)
         _, oi, _ = L.plan_dense(34, ops5, F)

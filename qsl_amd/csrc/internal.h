@@ -223,7 +223,9 @@ struct Tuning {
     int logt_f32 = 11, low_f32 = 3;
     int lds_max_target_f64 = 3;  // per-gate path: targets below use a tile
     int lds_max_target_f32 = 4;
-    int dense_f = 5;             // qubits per fused dense unitary (4 or 5)
+    int dense_f = 4;             // qubits per fused dense unitary (4 or 5); 4: half the
+                                 // matrix-core flops of 5 for 7% more unitaries (34q
+                                 // FP32 Haar layers: 11.8 s against 15.4 s at 5)
     int reg_tile = 1;            // fused passes hold amplitudes in registers
     int rthreads_f64 = 256;      // workgroup size of the register-tile kernel
     int rthreads_f32 = 256;
