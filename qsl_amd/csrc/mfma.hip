@@ -14,7 +14,9 @@
 //   D: accumulator j of lane l holds D[8*(j/4) + 4*(l/32) + j%4][l % 32].
 // Per amplitude a dense op costs 8 * 2^F real flops (256 for F = 5) against
 // 16 bytes of HBM traffic for the whole pass, so a pass holding two or more
-// 5-qubit unitaries is matrix-core bound.
+// 5-qubit unitaries is matrix-core bound.  F = 4 is the default: half the
+// flops per unitary, and layered circuits need only a few percent more
+// 4-qubit unitaries than 5-qubit ones.
 //
 // The A operand of every (m-tile, k-step) is laid out on the host as the 64
 // floats the 64 lanes need, so a wave fetches it with one coalesced 256-byte
