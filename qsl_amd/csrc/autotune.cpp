@@ -178,7 +178,7 @@ int qslhip_tuning_set(const char *key, int value)
             QH_REQUIRE(value >= 0 && value <= 3, QSLHIP_EINVAL, "pass_stream must be in [0, 3]");
             t.pass_stream = value;
         } else if (k == "np_hibonds") {
-            QH_REQUIRE(value == 0 || value == 1, QSLHIP_EINVAL, "np_hibonds must be 0 or 1");
+            QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "np_hibonds must be 0, 1 or 2");
             t.np_hibonds = value;
         } else if (k == "group_order") {
             QH_REQUIRE(value >= 0 && value <= 2, QSLHIP_EINVAL, "group_order must be 0, 1 or 2");

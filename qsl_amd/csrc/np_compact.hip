@@ -701,6 +701,92 @@ __global__ __launch_bounds__(kBlock) void k_np_hibonds(cplx<T> *__restrict__ amp
     }
 }
 
+// Up to kHiBondMax swap-like gates on pairwise disjoint pairs in the high
+// part (tuning np_hibonds 2), in one pass: the orbit of 2^a segments (a
+// active pairs) of k_np_hibonds, staged through LDS instead of registers,
+// X = kHiLds / 2^a elements of every member per round.  The gates run as
+// butterflies over the staged orbit in program order, a barrier between
+// gates: the same updates in the same order as k_np_hibonds, so the same
+// bits as one launch per gate.
+constexpr int kHiBondMax = 8;
+struct NPHiBonds {
+    int m;
+    int body[kHiBondMax];
+    uint32_t q0[kHiBondMax], q1[kHiBondMax];
+    double c[kHiBondMax], s[kHiBondMax];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_np_hibonds_lds(cplx<T> *__restrict__ amp, NPDev d,
+                                                           NPHiBonds B)
+{
+    constexpr uint32_t kHiLds = 32768 / sizeof(cplx<T>);  // 2^11 FP64, 2^12 FP32 amplitudes
+    constexpr int kHiLog = sizeof(cplx<T>) == 16 ? 11 : 12;
+    static_assert(kHiLds == (1u << kHiLog) && kHiLog > kHiBondMax, "staging size");
+    __shared__ cplx<T> sv[kHiLds];
+    __shared__ uint64_t soff[1 << kHiBondMax];
+    const Seg g = seg_of(d);
+    const uint64_t hix = (uint64_t)g.hi << d.L;
+    uint32_t active = 0;
+    for (int j = 0; j < B.m; j++) {
+        const bool a0 = (hix >> B.q0[j]) & 1ull, a1 = (hix >> B.q1[j]) & 1ull;
+        if (a0 != a1) {
+            if (!a0) return;  // another member of the orbit represents it
+            active |= 1u << j;
+        }
+    }
+    if (!active) return;
+    const int a = __popc(active);
+    // member t of the orbit: the segment with the pairs of t's set bits
+    // exchanged (bit k of t: the k-th active pair in program order)
+    if (threadIdx.x < (1u << a)) {
+        uint64_t y = hix;
+        int k = 0;
+        for (int j = 0; j < B.m; j++) {
+            if (!((active >> j) & 1u)) continue;
+            if ((threadIdx.x >> k) & 1u) y ^= (1ull << B.q0[j]) | (1ull << B.q1[j]);
+            k++;
+        }
+        soff[threadIdx.x] = threadIdx.x ? d.seg_off[y >> d.L] : g.off;
+    }
+    __syncthreads();
+    const int xb = kHiLog - a;  // log2 X
+    const uint32_t xm = (1u << xb) - 1;
+    const uint32_t end = g.i0 + kChunk < g.size ? g.i0 + kChunk : g.size;
+    for (uint32_t e0 = g.i0; e0 < end; e0 += xm + 1) {
+        // staged element idx: member idx >> xb, element e0 + (idx & xm)
+        for (uint32_t idx = threadIdx.x; idx < kHiLds; idx += kBlock) {
+            const uint32_t i = e0 + (idx & xm);
+            if (i < end) sv[idx] = amp[soff[idx >> xb] + i];
+        }
+        __syncthreads();
+        for (int j = 0; j < B.m; j++) {
+            if (!((active >> j) & 1u)) continue;
+            const int k = __popc(active & ((1u << j) - 1));
+            const T c = (T)B.c[j], s = (T)B.s[j];
+            const int body = B.body[j];
+            for (uint32_t p = threadIdx.x; p < kHiLds / 2; p += kBlock) {
+                const uint32_t el = p & xm;
+                if (e0 + el >= end) continue;
+                // member with bit k clear: pair j holds q0 = 1, q1 = 0 (the
+                // owner); its partner has bit k set
+                const uint32_t m0 = (uint32_t)ins0(p >> xb, (unsigned)k);
+                const uint32_t i0 = (m0 << xb) | el, i1 = ((m0 | (1u << k)) << xb) | el;
+                cplx<T> x0 = sv[i0], x1 = sv[i1];
+                pair_update<T>(body, x0, x1, c, s);
+                sv[i0] = x0;
+                sv[i1] = x1;
+            }
+            __syncthreads();
+        }
+        for (uint32_t idx = threadIdx.x; idx < kHiLds; idx += kBlock) {
+            const uint32_t i = e0 + (idx & xm);
+            if (i < end) amp[soff[idx >> xb] + i] = sv[idx];
+        }
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -1145,7 +1231,8 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
         // k_np_hibonds pass per group of up to kBondMax
         used = 0;
         j = i;
-        while (tuning().np_hibonds && j < gates.size() && j - i < (size_t)kBondMax &&
+        const size_t hmax = tuning().np_hibonds == 2 ? (size_t)kHiBondMax : (size_t)kBondMax;
+        while (tuning().np_hibonds && j < gates.size() && j - i < hmax &&
                bond_kind(gates[j].kind) && gates[j].q0 >= h->npt->L &&
                gates[j].q1 >= h->npt->L) {
             const uint64_t m = (1ull << gates[j].q0) | (1ull << gates[j].q1);
@@ -1286,6 +1373,24 @@ void np_apply_gates(qslhip_state *h, const std::vector<GateDesc> &gates)
             else
                 hipLaunchKernelGGL(k_np_lowbonds<float>, rgrid, blk, 0, h->stream,
                                    static_cast<cplx<float> *>(h->amps), d, h->npt->runblk, B);
+        } else if (pt.type == HIBONDS && pt.end - pt.first > (size_t)kBondMax) {
+            NPHiBonds B{};
+            B.m = (int)(pt.end - pt.first);
+            QH_REQUIRE(B.m <= kHiBondMax, QSLHIP_EINVAL, "NP high-bond group too large");
+            for (int j = 0; j < B.m; j++) {
+                const GateDesc &g = gates[pt.first + j];
+                B.body[j] = body_of(g.kind);
+                B.q0[j] = g.q0;
+                B.q1[j] = g.q1;
+                B.c[j] = g.c;
+                B.s[j] = g.s;
+            }
+            if (h->prec == 8)
+                hipLaunchKernelGGL(k_np_hibonds_lds<double>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<double> *>(h->amps), d, B);
+            else
+                hipLaunchKernelGGL(k_np_hibonds_lds<float>, grid, blk, 0, h->stream,
+                                   static_cast<cplx<float> *>(h->amps), d, B);
         } else {  // BONDS, HIBONDS
             NPBonds B{};
             B.m = (int)(pt.end - pt.first);

@@ -343,3 +343,40 @@ def test_np_fused_low_and_high_bond_passes(dtype, n, k):
         want = amp.astype(np.complex128)
         O.npc_apply_ops(want, n, k, ops, nthreads=8)
         assert _bits_equal(fused.getAmps(), want)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,k", [(32, 4), (32, 6), (30, 8), (28, 14)])
+def test_np_high_bond_orbits_staged_in_lds(dtype, n, k):
+    # groups of five to eight swap-like gates on disjoint pairs in the high
+    # part of the strings (tuning np_hibonds 2: one pass over orbits of up
+    # to 2^8 segments staged through LDS): bit-identical to one kernel per
+    # gate, for every swap-like kind and both orientations
+    rng = np.random.default_rng(1100 + n + k)
+    L = min(n, max(16, n - 16))
+    npairs = (n - L) // 2
+    seq = []
+    for rep in range(6):
+        hs = rng.permutation(np.arange(L, n))
+        for j in range(int(rng.integers(5, npairs + 1))):
+            seq.append((int(rng.choice([24, 26, 27, 28])), int(hs[2 * j]), int(hs[2 * j + 1]),
+                        0, _angle(rng, dtype)))
+        seq.append((4, int(rng.integers(n)), 0, 0, _angle(rng, dtype)))
+    ops = np.array(seq, OP_DTYPE)
+    total = math.comb(n, k)
+    amp = (rng.normal(size=total) + 1j * rng.normal(size=total)).astype(dtype)
+    amp /= np.linalg.norm(amp)
+    saved = qsl_amd._lib.tuning_get("np_hibonds")
+    try:
+        qsl_amd._lib.tuning_set("np_hibonds", 2)
+        fused = qsl_amd.QubitsNP(n, k, dtype=dtype)
+        fused.setAmps(amp)
+        fused.apply_ops(ops)
+        got = fused.getAmps()
+        del fused
+    finally:
+        qsl_amd._lib.tuning_set("np_hibonds", saved)
+    per_gate = qsl_amd.QubitsNP(n, k, dtype=dtype)
+    per_gate.setAmps(amp)
+    per_gate.apply_ops(ops, mode=qsl_amd.MODE_PER_GATE)
+    assert _bits_equal(got, per_gate.getAmps())
