@@ -238,7 +238,7 @@ struct Tuning {
     int np_hibonds = 2;          // NP: disjoint bonds in the high part of the strings
                                  // fused into one pass: 1 up to 4 (orbits in registers),
                                  // 2 up to 8 (groups of 5-8 staged through LDS: one
-                                 // pass per 32q layer instead of two, 2416 against 2182
+                                 // pass per 32q layer instead of two, 2518 against 2186
                                  // gates/s); 0 one pass per bond
     int group_order = 2;         // register-tile passes: 0 group g in workgroup slot g,
                                  // 1 each XCD sweeping an eighth of the groups, 2 the

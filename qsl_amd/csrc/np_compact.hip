@@ -707,7 +707,8 @@ __global__ __launch_bounds__(kBlock) void k_np_hibonds(cplx<T> *__restrict__ amp
 // X = kHiLds / 2^a elements of every member per round.  The gates run as
 // butterflies over the staged orbit in program order, a barrier between
 // gates: the same updates in the same order as k_np_hibonds, so the same
-// bits as one launch per gate.
+// bits as one launch per gate.  The next round's elements are loaded into
+// registers while a round's bonds run.
 constexpr int kHiBondMax = 8;
 struct NPHiBonds {
     int m;
@@ -753,13 +754,31 @@ __global__ __launch_bounds__(kBlock) void k_np_hibonds_lds(cplx<T> *__restrict__
     const int xb = kHiLog - a;  // log2 X
     const uint32_t xm = (1u << xb) - 1;
     const uint32_t end = g.i0 + kChunk < g.size ? g.i0 + kChunk : g.size;
+    // staged element idx = threadIdx.x + r * kBlock: member idx >> xb,
+    // element e0 + (idx & xm).  The next round's loads are issued once this
+    // round is in LDS, so they land during its bonds and stores.
+    constexpr int kPer = kHiLds / kBlock;
+    cplx<T> nx[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+        const uint32_t idx = threadIdx.x + r * kBlock, i = g.i0 + (idx & xm);
+        nx[r] = i < end ? amp[soff[idx >> xb] + i] : cplx<T>{};
+    }
     for (uint32_t e0 = g.i0; e0 < end; e0 += xm + 1) {
-        // staged element idx: member idx >> xb, element e0 + (idx & xm)
-        for (uint32_t idx = threadIdx.x; idx < kHiLds; idx += kBlock) {
-            const uint32_t i = e0 + (idx & xm);
-            if (i < end) sv[idx] = amp[soff[idx >> xb] + i];
+#pragma unroll
+        for (int r = 0; r < kPer; r++) {
+            const uint32_t idx = threadIdx.x + r * kBlock;
+            if (e0 + (idx & xm) < end) sv[idx] = nx[r];
         }
         __syncthreads();
+        const uint32_t e1 = e0 + xm + 1;
+        if (e1 < end) {
+#pragma unroll
+            for (int r = 0; r < kPer; r++) {
+                const uint32_t idx = threadIdx.x + r * kBlock, i = e1 + (idx & xm);
+                nx[r] = i < end ? amp[soff[idx >> xb] + i] : cplx<T>{};
+            }
+        }
         for (int j = 0; j < B.m; j++) {
             if (!((active >> j) & 1u)) continue;
             const int k = __popc(active & ((1u << j) - 1));
