@@ -189,7 +189,10 @@ int qslhip_jit_wait(void);
  * Keys: "logt_f64", "low_f64", "logt_f32", "low_f32" (LDS tile width and the
  * contiguous low run of fused passes), "lds_max_target_f64/_f32" (highest
  * target the per-gate path routes through a one-op tile; -1 = none),
- * "dense_f" (qubits per fused dense unitary, 4 or 5).  qslhip_autotune times
+ * "dense_f" (qubits per fused dense unitary, 4 (default) or 5).  Further
+ * keys select kernel variants (number-preserving bond passes "np_lowbonds",
+ * "np_hibonds" 0-2, "np_bonds"; launch shapes; streaming); an unknown key or
+ * an out-of-range value returns QSLHIP_EINVAL.  qslhip_autotune times
  * the candidates on the device (scratch state of min(nqubits, 26) qubits),
  * keeps the fastest in the process-wide table and writes a JSON report. */
 int qslhip_tuning_get(const char *key, int *value);
