@@ -315,7 +315,17 @@ __global__ __launch_bounds__(kBlock) void k_subs_from_runs(const double *__restr
     extern __shared__ double blk[];
     const uint32_t nb = 8u << L.lb;
     const double *src = runs + (uint64_t)blockIdx.x * nb;
-    for (uint32_t i = threadIdx.x; i < nb; i += kBlock) blk[i] = src[i];
+    // eight loads in flight per thread before the first LDS write
+    constexpr uint32_t kU = 8;
+    uint32_t i = threadIdx.x;
+    for (; i + (kU - 1) * kBlock < nb; i += kU * kBlock) {
+        double t[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) t[u] = src[i + u * kBlock];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) blk[i + u * kBlock] = t[u];
+    }
+    for (; i < nb; i += kBlock) blk[i] = src[i];
     __syncthreads();
     const uint32_t lmask = (1u << L.lb) - 1;
     for (uint32_t u = threadIdx.x; u <= lmask; u += kBlock) {
